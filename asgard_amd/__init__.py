@@ -1,0 +1,347 @@
+"""asgard_amd: MI355X (gfx950) kronmult operator apply and implicit-step GMRES.
+
+Python mirror of the reference call surface for this path, bound to the C ABI
+of ``libasgard_amd.so`` (declared in ``include/asgard_amd.h``):
+
+* :class:`KronmultMatrix` -- ``asgard::kronmult_matrix<P>``
+  (src/asgard_kronmult_matrix.hpp:111-682): sparse, sparse-grid and dense
+  constructors, ``apply(alpha, x, beta, y)``, sizes and flops,
+  ``update_stored_coefficients``;
+* :func:`simple_gmres_euler` -- ``asgard::solver::simple_gmres_euler``
+  (src/solver.cpp:119-135) returning ``GmresInfo(error, iterations)``
+  (src/pde/pde_base.hpp:94-98);
+* :func:`connect_1d` and :func:`sparse_pattern` -- the index construction of
+  ``connect_1d`` (src/asgard_grid_1d.hpp) and ``check_connected`` scans
+  (src/asgard_kronmult_matrix.cpp:918-938), the latter on the GPU.
+
+Vectors are torch CUDA tensors (device pointers, ``apply<resource::device>``)
+or numpy arrays (host pointers, ``apply<resource::host>``).  There is no CPU
+fallback: the HIP library must be built and a GPU present for any compute
+call; errors raise :class:`AkmError` (``ValueError`` for invalid arguments,
+as the reference throws ``std::invalid_argument``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from collections import namedtuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libasgard_amd.so")
+
+AKM_SUCCESS = 0
+AKM_ERR_INVALID = -1
+AKM_ERR_UNSUPPORTED = -2
+AKM_ERR_HIP = -3
+AKM_ERR_MEMORY = -4
+AKM_ERR_NOT_CONVERGED = 1
+AKM_F64 = 0
+AKM_F32 = 1
+
+#: every symbol include/asgard_amd.h declares
+EXPORTS = (
+    "akm_last_error", "akm_version", "akm_device_count", "akm_set_stream",
+    "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
+    "akm_create_sparse_grid", "akm_create_dense", "akm_update_coefficients",
+    "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
+    "akm_time_apply", "akm_destroy", "akm_gmres_euler",
+)
+
+GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
+
+
+class AkmError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[akm {code}] {message}")
+        self.code = code
+
+
+class AkmInvalid(AkmError, ValueError):
+    pass
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libasgard_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"asgard_amd: {path} is missing; build it first (__graft_entry__.build() "
+            "or make -C asgard_amd/csrc)")
+    L = ctypes.CDLL(path)
+    L.akm_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.akm_last_error.restype = _i
+    L.akm_version.restype = ctypes.c_char_p
+    L.akm_device_count.argtypes = [ctypes.POINTER(_i)]
+    L.akm_set_stream.argtypes = [_vp, _vp]
+    L.akm_synchronize.argtypes = [_vp]
+    L.akm_connect_1d.argtypes = [_i, _i, _vp, _vp, _vp, ctypes.POINTER(_i)]
+    L.akm_sparse_pattern.argtypes = [_i, _i64, _vp, _vp, _vp]
+    L.akm_create_sparse.argtypes = [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i64, _i,
+                                    ctypes.POINTER(_vp)]
+    L.akm_create_sparse_grid.argtypes = [_i, _i, _i64, _vp, _i, _i, _vp, _i64, _i,
+                                         ctypes.POINTER(_vp)]
+    L.akm_create_dense.argtypes = [_i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i,
+                                   ctypes.POINTER(_vp)]
+    L.akm_update_coefficients.argtypes = [_vp, _vp]
+    L.akm_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i]
+    L.akm_apply_euler.argtypes = [_vp, _d, _d, _vp, _d, _vp]
+    L.akm_matrix_info.argtypes = [_vp] + [ctypes.POINTER(_i64)] * 5
+    L.akm_set_tuning.argtypes = [_vp, _i64, _i]
+    L.akm_time_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i, ctypes.POINTER(_d)]
+    L.akm_destroy.argtypes = [_vp]
+    L.akm_gmres_euler.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
+                                  ctypes.POINTER(_i)]
+    for name in EXPORTS:
+        if name not in ("akm_last_error", "akm_version"):
+            getattr(L, name).restype = _i
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    L = load_library()
+    buf = ctypes.create_string_buffer(1024)
+    L.akm_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def _check(rc, allow_not_converged=False):
+    if rc == AKM_SUCCESS or (allow_not_converged and rc == AKM_ERR_NOT_CONVERGED):
+        return rc
+    msg = last_error()
+    if rc == AKM_ERR_INVALID:
+        raise AkmInvalid(rc, msg)
+    raise AkmError(rc, msg)
+
+
+def device_count() -> int:
+    c = _i(0)
+    _check(load_library().akm_device_count(ctypes.byref(c)))
+    return c.value
+
+
+# ---------------------------------------------------------------- vectors ---
+
+def _is_torch(t):
+    return type(t).__module__.startswith("torch")
+
+
+def _vec_ptr(v, dtype):
+    """(pointer, on_device) of a torch CUDA tensor or numpy array of `dtype`."""
+    if _is_torch(v):
+        import torch
+        want = torch.float64 if dtype == np.float64 else torch.float32
+        if v.dtype != want or not v.is_contiguous():
+            raise AkmInvalid(AKM_ERR_INVALID, f"vector must be a contiguous {want} tensor")
+        if not v.is_cuda:
+            raise AkmInvalid(AKM_ERR_INVALID, "torch vectors must live on the GPU")
+        return v.data_ptr(), 1
+    if not isinstance(v, np.ndarray) or v.dtype != dtype or not v.flags.c_contiguous:
+        raise AkmInvalid(AKM_ERR_INVALID, f"vector must be a contiguous {np.dtype(dtype)} array")
+    return v.ctypes.data, 0
+
+
+def _np(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ------------------------------------------------------------ index build ---
+
+def connect_1d(max_level, full=True):
+    """(pntr, indx, diag) of the 1D cell hierarchy connectivity."""
+    L = load_library()
+    nconn = _i(0)
+    _check(L.akm_connect_1d(int(max_level), int(full), None, None, None, ctypes.byref(nconn)))
+    rows = 1 << int(max_level)
+    pntr = np.zeros(rows + 1, np.int32)
+    indx = np.zeros(nconn.value, np.int32)
+    diag = np.zeros(rows, np.int32)
+    _check(L.akm_connect_1d(int(max_level), int(full), pntr.ctypes.data, indx.ctypes.data,
+                            diag.ctypes.data, ctypes.byref(nconn)))
+    return pntr, indx, diag
+
+
+def sparse_pattern(table):
+    """Cell-pair CSR (pntr int64, indx int32) computed on the GPU."""
+    L = load_library()
+    table = _np(table, np.int32)
+    cells, twod = table.shape
+    pntr = np.zeros(cells + 1, np.int64)
+    _check(L.akm_sparse_pattern(twod // 2, cells, table.ctypes.data, pntr.ctypes.data, None))
+    indx = np.zeros(int(pntr[-1]), np.int32)
+    _check(L.akm_sparse_pattern(twod // 2, cells, table.ctypes.data, pntr.ctypes.data,
+                                indx.ctypes.data))
+    return pntr, indx
+
+
+# ----------------------------------------------------------------- matrix ---
+
+class KronmultMatrix:
+    """Device-resident ``kronmult_matrix<P>`` (P = float64 or float32)."""
+
+    def __init__(self, handle, prec, d, n, rows, cols, num_terms):
+        self._h = handle
+        self.prec = prec
+        self.dtype = np.float64 if prec == AKM_F64 else np.float32
+        self.num_dimensions = d
+        self.kron_size = n
+        self.num_rows = rows
+        self.num_cols = cols
+        self.num_terms = num_terms
+
+    # ---- constructors
+    @classmethod
+    def sparse(cls, d, n, rows, cols, num_terms, pntr, indx, iA, vA, dtype=np.float64):
+        """Sparse single-call constructor (hpp:235-272, CPU CSR format)."""
+        L = load_library()
+        prec = AKM_F64 if np.dtype(dtype) == np.float64 else AKM_F32
+        pntr = _np(pntr, np.int64)
+        indx = _np(indx, np.int32)
+        iA = _np(iA, np.int32)
+        vA = _np(vA, dtype)
+        h = _vp()
+        _check(L.akm_create_sparse(d, n, rows, cols, num_terms, pntr.ctypes.data,
+                                   indx.ctypes.data, iA.ctypes.data, vA.ctypes.data, vA.size,
+                                   prec, ctypes.byref(h)))
+        return cls(h, prec, d, n, rows, cols, num_terms)
+
+    @classmethod
+    def sparse_grid(cls, d, n, table, max_level, num_terms, vA, dtype=np.float64):
+        """make_kronmult_sparse equivalent from the element table."""
+        L = load_library()
+        prec = AKM_F64 if np.dtype(dtype) == np.float64 else AKM_F32
+        table = _np(table, np.int32)
+        vA = _np(vA, dtype)
+        h = _vp()
+        _check(L.akm_create_sparse_grid(d, n, table.shape[0], table.ctypes.data,
+                                        int(max_level), num_terms, vA.ctypes.data, vA.size,
+                                        prec, ctypes.byref(h)))
+        return cls(h, prec, d, n, table.shape[0], table.shape[0], num_terms)
+
+    @classmethod
+    def dense(cls, d, n, rows, cols, num_terms, terms, elem, row_offset, col_offset,
+              num_1d_blocks, dtype=np.float64):
+        """Dense constructor (hpp:132-176)."""
+        L = load_library()
+        prec = AKM_F64 if np.dtype(dtype) == np.float64 else AKM_F32
+        terms = [_np(t, dtype) for t in terms]
+        elem = _np(elem, np.int32).reshape(-1)
+        ptrs = (ctypes.c_void_p * max(1, len(terms)))(*[t.ctypes.data for t in terms])
+        h = _vp()
+        _check(L.akm_create_dense(d, n, rows, cols, num_terms, ptrs, elem.ctypes.data,
+                                  elem.size // d, row_offset, col_offset, num_1d_blocks, prec,
+                                  ctypes.byref(h)))
+        return cls(h, prec, d, n, rows, cols, num_terms)
+
+    # ---- lifetime
+    def close(self):
+        if self._h:
+            load_library().akm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __bool__(self):
+        return bool(self._h) and self.num_dimensions > 0
+
+    # ---- sizes
+    def _info(self):
+        vals = [_i64() for _ in range(5)]
+        _check(load_library().akm_matrix_info(self._h, *[ctypes.byref(v) for v in vals]))
+        return [v.value for v in vals]
+
+    def tensor_size(self):
+        return self._info()[0]
+
+    def input_size(self):
+        return self._info()[1]
+
+    def output_size(self):
+        return self._info()[2]
+
+    def num_pairs(self):
+        return self._info()[3]
+
+    def flops(self):
+        """Flops per apply, ×2 convention (kronmult_matrix::compute_flops)."""
+        return self._info()[4]
+
+    # ---- compute
+    def set_stream(self, stream_ptr):
+        _check(load_library().akm_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        _check(load_library().akm_synchronize(self._h))
+
+    def set_tuning(self, chunk_columns=0, force_generic=False):
+        _check(load_library().akm_set_tuning(self._h, int(chunk_columns), int(force_generic)))
+
+    def apply(self, alpha, x, beta, y):
+        """y = alpha*A*x + beta*y (device tensors or host arrays)."""
+        px, dx = _vec_ptr(x, self.dtype)
+        py, dy = _vec_ptr(y, self.dtype)
+        if dx != dy:
+            raise AkmInvalid(AKM_ERR_INVALID, "x and y must both be host or both device")
+        _check(load_library().akm_apply(self._h, float(alpha), px, float(beta), py, dx))
+        return y
+
+    def apply_euler(self, dt, alpha, x, beta, y):
+        """y = alpha*(x - dt*A*x) + beta*y on device tensors."""
+        px, dx = _vec_ptr(x, self.dtype)
+        py, dy = _vec_ptr(y, self.dtype)
+        if not (dx and dy):
+            raise AkmInvalid(AKM_ERR_INVALID, "apply_euler needs device tensors")
+        _check(load_library().akm_apply_euler(self._h, float(dt), float(alpha), px, float(beta),
+                                              py))
+        return y
+
+    def time_apply(self, alpha, x, beta, y, iterations):
+        px, dx = _vec_ptr(x, self.dtype)
+        py, dy = _vec_ptr(y, self.dtype)
+        if not (dx and dy):
+            raise AkmInvalid(AKM_ERR_INVALID, "time_apply needs device tensors")
+        ms = _d()
+        _check(load_library().akm_time_apply(self._h, float(alpha), px, float(beta), py,
+                                             int(iterations), ctypes.byref(ms)))
+        return ms.value
+
+    def update_stored_coefficients(self, values):
+        if isinstance(values, (list, tuple)):
+            arrs = [_np(v, self.dtype) for v in values]
+            ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+            _check(load_library().akm_update_coefficients(self._h, ptrs))
+        else:
+            arr = _np(values, self.dtype)
+            _check(load_library().akm_update_coefficients(self._h, arr.ctypes.data))
+
+
+def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, tol=-1.0):
+    """Solves (I - dt*A) x = b in place; returns GmresInfo(error, iterations).
+
+    restart/max_iter/tol <= 0 select the reference defaults (parser::NO_USER_VALUE).
+    Raises ValueError when restart exceeds the system size."""
+    px, dx = _vec_ptr(x, mat.dtype)
+    pb, db = _vec_ptr(b, mat.dtype)
+    if dx != db:
+        raise AkmInvalid(AKM_ERR_INVALID, "x and b must both be host or both device")
+    err = _d()
+    its = _i()
+    _check(load_library().akm_gmres_euler(mat._h, float(dt), px, pb, int(restart),
+                                          int(max_iter), float(tol), dx, ctypes.byref(err),
+                                          ctypes.byref(its)), allow_not_converged=True)
+    return GmresInfo(err.value, its.value)

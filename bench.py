@@ -1,0 +1,153 @@
+"""Benchmark: FP64 sparse-grid kronmult apply on one MI355X.
+
+Workload (one step = one apply y = A x over the whole grid, inputs resident
+in HBM): d = 3, n = 3 (degree 2), |l|_1 <= 6 sparse grid (688 cells, 18,576
+unknowns), 352,768 connected cell pairs, 3 terms with U[-1, 1] 1D operators of
+size 192 x 192 per dimension, x ~ N(0, 1), seed 0.
+
+Reported: GFLOP/s in the x2 convention of kronmult_matrix::compute_flops
+(2*d*n^(d+1)*T per connected pair and term), the roofline of the kronmult
+kernel (FP64 peak 78.6 TFLOP/s), the FP64 parity of the GPU result against
+the CPU oracle, and the oracle's OpenMP kronmult (a restatement of the
+reference's cpu_sparse) timed on this host's cores on the same apply.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(d, n, L, T, seed=0):
+    import oracle  # CPU input generator for the grid and the 1D operators
+
+    rng = np.random.default_rng(seed)
+    table, _ = oracle.element_table([L] * d)
+    size1d = n * (1 << L)
+    ops = [np.asfortranarray(rng.uniform(-1, 1, (size1d, size1d))) for _ in range(T * d)]
+    vA = oracle.build_vA(d, n, T, L, ops)
+    x = rng.standard_normal(table.shape[0] * n ** d)
+    return table, vA, x
+
+
+def algorithmic_bytes(d, n, T, cells, nnz, nconn):
+    """Unique bytes an apply must touch: x, y, CSR, cell 1D indices, vA."""
+    dof = cells * n ** d
+    return 8 * dof + 8 * dof + 4 * (cells + 1 + nnz) + 4 * cells * d + 8 * n * n * d * T * nconn
+
+
+def cpu_baseline(d, n, table, vA, x, T, L, ref_y_gpu):
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_num_threads(threads)
+    pntr, indx, iA = oracle.sparse_pattern(table, n=n, T=T, max_level=L)
+    y = np.zeros_like(x)
+    oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, y)  # warm-up
+    times = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, y)
+        times.append(time.perf_counter() - t0)
+    flops = 2 * d * n ** (d + 1) * T * int(pntr[-1])
+    med = float(np.median(times))
+    err = float(np.abs(ref_y_gpu - y).max() / np.abs(y).max())
+    return {"value": flops / med * 1e-9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": "full apply of the same workload, median of 20 after 1 warm-up "
+                      "(OpenMP restatement of the reference's cpu_sparse)"}, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (the row-sharded multi-GPU design is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import asgard_amd
+
+    d, n, L, T = 3, 3, 6, 3
+    table, vA, x = make_workload(d, n, L, T)
+    cells = table.shape[0]
+    mat = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
+    stream = torch.cuda.current_stream()
+    mat.set_stream(stream.cuda_stream)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    flops = mat.flops()
+    nnz = mat.num_pairs()
+    nconn = len(asgard_amd.connect_1d(L)[1])
+
+    for _ in range(args.warmup):
+        mat.apply(1.0, xd, 0.0, yd)
+    torch.cuda.synchronize()
+
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        mat.apply(1.0, xd, 0.0, yd)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = start.elapsed_time(stop)
+    ms_per_step = ev_ms / args.steps
+    gflops = flops / (ms_per_step * 1e-3) * 1e-9
+
+    y_gpu = yd.cpu().numpy()
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(d, n, table, vA, x, T, L, y_gpu)
+
+    achieved_tflops = gflops * 1e-3
+    line = {
+        "metric": "kronmult GFLOP/s (FP64 apply)",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "wall_s": round(wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "kronmult apply, d=3 n=3 |l|_1<=6 sparse grid, 3 terms",
+                   "cells": int(cells), "connected_pairs": int(nnz),
+                   "flops_per_apply": int(flops), "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None,
+                     "algorithmic_bytes": int(algorithmic_bytes(d, n, T, cells, nnz, nconn))},
+        "cpu_baseline": cpu,
+        "parity_max_rel_err_vs_oracle": parity,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

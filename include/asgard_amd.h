@@ -1,0 +1,173 @@
+/*
+ * asgard_amd: MI355X (gfx950) kronmult operator apply and implicit-step GMRES.
+ *
+ * C ABI.  Plain pointers and sizes; no exceptions cross it.  Every entry point
+ * returns an akm_status (0 on success); akm_last_error() describes the most
+ * recent failure of the calling thread.
+ *
+ * Each declaration names the reference interface (project-asgard/asgard) it
+ * replaces.  Precision: a matrix is created in FP64 or FP32 (akm_prec) and
+ * every x, y, b, vA pointer passed with it has that element type; scalars are
+ * passed as double and converted.
+ *
+ * Vectors: "on_device" != 0 means x/y/b are device pointers (the reference's
+ * apply<resource::device>); 0 means host pointers, copied in and out through
+ * device workspaces owned by the matrix, exactly as the reference's
+ * apply<resource::host> does with its xdev/ydev workspaces
+ * (src/asgard_kronmult_matrix.hpp:351-358, 451-452).
+ *
+ * Threading: a matrix is not thread-safe (the reference's apply is not,
+ * src/asgard_kronmult_matrix.hpp:336).  Work is enqueued on the matrix's
+ * stream (default: the null stream) and is asynchronous for device pointers.
+ */
+#ifndef ASGARD_AMD_H
+#define ASGARD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int akm_status;
+#define AKM_SUCCESS 0
+#define AKM_ERR_INVALID -1     /* invalid argument (std::invalid_argument) */
+#define AKM_ERR_UNSUPPORTED -2 /* unsupported (d, n) (std::runtime_error) */
+#define AKM_ERR_HIP -3         /* HIP runtime failure */
+#define AKM_ERR_MEMORY -4      /* device allocation failure */
+#define AKM_ERR_NOT_CONVERGED 1 /* GMRES stopped above the tolerance (info) */
+
+typedef enum
+{
+  AKM_F64 = 0,
+  AKM_F32 = 1
+} akm_prec;
+
+typedef struct akm_matrix_s *akm_matrix;
+
+/* ------------------------------------------------------------ utilities -- */
+
+/* Copies the calling thread's last error message; returns its length. */
+int akm_last_error(char *buffer, size_t size);
+
+/* Library version string. */
+const char *akm_version(void);
+
+/* Number of visible HIP devices (0 when none). */
+akm_status akm_device_count(int *count);
+
+/* Selects the stream the matrix enqueues work on (a hipStream_t, NULL = the
+ * null stream).  The reference always used the default stream. */
+akm_status akm_set_stream(akm_matrix mat, void *stream);
+
+/* Blocks until the matrix's stream is idle. */
+akm_status akm_synchronize(akm_matrix mat);
+
+/* --------------------------------------------------- index construction -- */
+
+/* connect_1d(max_level, full ? hierarchy::full : hierarchy::volume)
+ * (src/asgard_grid_1d.hpp:34-46, 159-317): CSR of the 1D cell hierarchy.
+ * pntr has 2^max_level + 1 entries, indx *num_connections, diag 2^max_level;
+ * call with NULL arrays to query num_connections.  Host arrays. */
+akm_status akm_connect_1d(int max_level, int full, int *pntr, int *indx, int *diag,
+                          int *num_connections);
+
+/* Cell-pair sparsity of the kronmult operator on the GPU: replaces the
+ * O(cells^2) check_connected scans of compute_mem_usage and
+ * make_kronmult_sparse (src/asgard_kronmult_matrix.cpp:918-938, 411-466).
+ * table is the flattened element table (num_cells x 2d host ints:
+ * levels then cell positions, elements::table::get_active_table()).
+ * Writes pntr (num_cells + 1, int64, host) and, when indx is not NULL, the
+ * column cell ids (host, nnz entries, ascending in each row).  Call once with
+ * indx = NULL to get nnz = pntr[num_cells], then again with indx. */
+akm_status akm_sparse_pattern(int d, int64_t num_cells, const int *table, int64_t *pntr,
+                              int *indx);
+
+/* ----------------------------------------------------- matrix lifecycle -- */
+
+/* Sparse single-call constructor of kronmult_matrix<P>
+ * (src/asgard_kronmult_matrix.hpp:235-272, CPU format): pntr (rows + 1) and
+ * indx (nnz) are the CSR of column cell ids, iA (nnz*T*d) the factor offsets
+ * into vA (index (pair*T + t)*d + dim, dim 0 applied to the slowest tensor
+ * index), vA (num_vA values) the n x n column-major blocks.  All host arrays;
+ * the matrix copies them to the device (the reference moves them in). */
+akm_status akm_create_sparse(int d, int n, int rows, int cols, int num_terms,
+                             const int64_t *pntr, const int *indx, const int *iA,
+                             const void *vA, int64_t num_vA, akm_prec prec,
+                             akm_matrix *mat);
+
+/* Sparse-grid factory: make_kronmult_sparse (src/asgard_kronmult_matrix.cpp:330-690)
+ * without the per-pair iA list.  From the element table (num_cells x 2d host
+ * ints) it builds the cell-pair CSR on the GPU and derives every factor offset
+ * on the fly from the cells' 1D indices and connect_1d(max_level).  vA is the
+ * reference's block array: for each connect_1d(max_level) connection, for
+ * each term, for each dim, one n x n column-major block (num_connections*d*T*n*n
+ * values, as make_kronmult_sparse fills it at :369-393).  Host arrays. */
+akm_status akm_create_sparse_grid(int d, int n, int64_t num_cells, const int *table,
+                                  int max_level, int num_terms, const void *vA,
+                                  int64_t num_vA, akm_prec prec, akm_matrix *mat);
+
+/* Dense constructor of kronmult_matrix<P>
+ * (src/asgard_kronmult_matrix.hpp:132-176): every row/column pair, term t's
+ * factors in terms[t] (d*nb*nb*n*n values each, block (ocol, orow) of dim k at
+ * k*nb*nb*n*n + n*n*(ocol*nb + orow)), elem the cells' 1D indices
+ * (num_indexes x d), row/col offsets into elem.  Host arrays. */
+akm_status akm_create_dense(int d, int n, int rows, int cols, int num_terms,
+                            const void *const *terms, const int *elem, int num_indexes,
+                            int row_offset, int col_offset, int num_1d_blocks,
+                            akm_prec prec, akm_matrix *mat);
+
+/* update_stored_coefficients (src/asgard_kronmult_matrix.hpp:520-565):
+ * replaces vA (sparse) or the term arrays (dense; terms is then a
+ * const void *const * of num_terms arrays) keeping the index lists. */
+akm_status akm_update_coefficients(akm_matrix mat, const void *values);
+
+/* kronmult_matrix<P>::apply<resource>(alpha, x, beta, y)
+ * (src/asgard_kronmult_matrix.hpp:340-478): y = alpha*A*x + beta*y.
+ * With no terms: y = beta*y.  beta == 0 never reads y. */
+akm_status akm_apply(akm_matrix mat, double alpha, const void *x, double beta, void *y,
+                     int on_device);
+
+/* The implicit operator of simple_gmres_euler (src/solver.cpp:126-131) in
+ * one pass: y = alpha*(x - dt*A*x) + beta*y (device pointers). */
+akm_status akm_apply_euler(akm_matrix mat, double dt, double alpha, const void *x,
+                           double beta, void *y);
+
+/* Sizes: tensor_size n^d, input_size, output_size (hpp:481-490), number of
+ * connected pairs, and flops per apply in the ×2 convention of
+ * kronmult_matrix::compute_flops (hpp:507-512). */
+akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_size,
+                           int64_t *output_size, int64_t *num_pairs, int64_t *flops);
+
+/* Tuning knob: columns per work chunk (0 = automatic) and whether to force the
+ * block-wide LDS kernel (for tests).  Takes effect at the next apply. */
+akm_status akm_set_tuning(akm_matrix mat, int64_t chunk_columns, int force_generic);
+
+/* Times `iterations` back-to-back applies (device x/y) with HIP events on the
+ * matrix's stream; returns the average milliseconds per apply. */
+akm_status akm_time_apply(akm_matrix mat, double alpha, const void *x, double beta,
+                          void *y, int iterations, double *ms_per_apply);
+
+/* Releases the matrix and all of its device memory. */
+akm_status akm_destroy(akm_matrix mat);
+
+/* ---------------------------------------------------------------- GMRES -- */
+
+/* solver::simple_gmres_euler<P, resource> (src/solver.cpp:119-135) on top of
+ * solver::simple_gmres (:156-298): solves (I - dt*A) x = b with restarted
+ * GMRES(restart), classical Gram-Schmidt, Givens rotations and an absolute
+ * tolerance on the residual estimate, x holding the initial guess on entry.
+ * restart/max_iter/tol <= 0 select the reference defaults
+ * (default_gmres_restarts, n outer iterations, 1e-12 FP64 / 1e-6 FP32).
+ * All vector work stays on the GPU.  err and iterations mirror gmres_info.
+ * Returns AKM_ERR_INVALID if restart > n, as the reference throws. */
+akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, int restart,
+                           int max_iter, double tol, int on_device, double *err,
+                           int *iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -70,6 +70,9 @@ def lib():
         L.orc_sparse_pattern_count.argtypes = [_c_int, _c_i64, _i32p, _i64p]
         L.orc_sparse_pattern_fill.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_i64,
                                               _i32p, _i64p, _i32p, _vp]
+        L.orc_sparse_pattern_rows.restype = _c_i64
+        L.orc_sparse_pattern_rows.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_i64, _i32p,
+                                              _i32p, _c_int, _i64p, _vp, _vp]
         L.orc_cell_1d_index.argtypes = [_c_int, _c_i64, _i32p, _i32p]
         L.orc_build_vA.argtypes = [_c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _f64p]
         L.orc_build_dense_terms.argtypes = [_c_int, _c_int, _c_int, _c_int, _vp,
@@ -187,6 +190,29 @@ def sparse_pattern(table, n=2, T=1, max_level=None, with_iA=True):
                                        pntr, indx, iA.ctypes.data if with_iA else None)
     if rc != 0:
         raise RuntimeError(f"sparse_pattern_fill failed ({rc})")
+    return pntr, indx, iA
+
+
+def sparse_pattern_rows(table, rows, n=2, T=1, max_level=None):
+    """CSR (pntr, indx) and iA of the selected row cells only."""
+    table = _i32(table)
+    cells, twod = table.shape
+    d = twod // 2
+    if max_level is None:
+        max_level = int(table[:, :d].max())
+    max_level = max(int(max_level), 1)
+    rows = _i32(rows)
+    pntr = np.zeros(len(rows) + 1, dtype=np.int64)
+    nnz = lib().orc_sparse_pattern_rows(d, n, T, max_level, cells, table.reshape(-1), rows,
+                                        len(rows), pntr, None, None)
+    if nnz < 0:
+        raise RuntimeError("pattern rows failed")
+    indx = np.zeros(nnz, dtype=np.int32)
+    iA = np.zeros(nnz * T * d, dtype=np.int32)
+    rc = lib().orc_sparse_pattern_rows(d, n, T, max_level, cells, table.reshape(-1), rows,
+                                       len(rows), pntr, indx.ctypes.data, iA.ctypes.data)
+    if rc < 0:
+        raise RuntimeError("pattern rows failed")
     return pntr, indx, iA
 
 

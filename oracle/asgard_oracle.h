@@ -71,6 +71,14 @@ int orc_sparse_pattern_fill(int d, int n, int T, int max_level, int64_t num_cell
                             const int *table, const int64_t *pntr, int *indx,
                             int *iA);
 
+/* Same as the fill above for a subset of row cells only (rows[num_sel]):
+ * pntr (num_sel + 1), indx and iA of those rows; indx/iA may be NULL to
+ * count.  Returns the number of pairs or a negative code. */
+int64_t orc_sparse_pattern_rows(int d, int n, int T, int max_level,
+                                int64_t num_cells, const int *table,
+                                const int *rows, int num_sel, int64_t *pntr,
+                                int *indx, int *iA);
+
 /* 1D index of every cell and dimension: elem[c*d + k] = 0 if l == 0 else
  * 2^(l-1) + p  (src/asgard_kronmult_matrix.cpp:121-132). */
 void orc_cell_1d_index(int d, int64_t num_cells, const int *table, int *elem);

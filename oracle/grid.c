@@ -577,3 +577,55 @@ void orc_build_dense_terms(int d, int n, int T, int nb,
     }
   }
 }
+
+int64_t orc_sparse_pattern_rows(int d, int n, int T, int max_level,
+                                int64_t num_cells, const int *table,
+                                const int *rows, int num_sel, int64_t *pntr,
+                                int *indx, int *iA)
+{
+  /* the rows of make_kronmult_sparse (src/asgard_kronmult_matrix.cpp:406-466)
+   * restricted to the selected row cells; indx/iA may be NULL to count */
+  int nconn = orc_connect_1d(max_level, 1, NULL, NULL, NULL);
+  if (nconn < 0)
+    return -1;
+  int *cp = (int *)malloc(sizeof(int) * ((size_t)(1 << max_level) + 1));
+  int *ci = (int *)malloc(sizeof(int) * (size_t)nconn);
+  orc_connect_1d(max_level, 1, cp, ci, NULL);
+  const int n2 = n * n;
+  const int block1d = d * T * n2;
+  int64_t j = 0;
+  int bad = 0;
+  pntr[0] = 0;
+  for (int s = 0; s < num_sel; s++)
+  {
+    const int *rc = table + 2 * d * (int64_t)rows[s];
+    for (int64_t col = 0; col < num_cells; col++)
+    {
+      const int *cc = table + 2 * d * col;
+      if (!orc_check_connected(d, rc, cc))
+        continue;
+      if (indx)
+        indx[j] = (int)col;
+      if (iA)
+      {
+        int off[16];
+        for (int k = 0; k < d; k++)
+        {
+          off[k] = orc_connect_1d_offset(cp, ci, cell_1d(rc[k], rc[d + k]),
+                                         cell_1d(cc[k], cc[d + k]));
+          if (off[k] < 0)
+            bad++;
+        }
+        int *ia = iA + j * (int64_t)T * d;
+        for (int t = 0; t < T; t++)
+          for (int k = 0; k < d; k++)
+            *ia++ = off[k] * block1d + (t * d + k) * n2;
+      }
+      j++;
+    }
+    pntr[s + 1] = j;
+  }
+  free(cp);
+  free(ci);
+  return bad ? -2 : j;
+}
