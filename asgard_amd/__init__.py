@@ -46,7 +46,7 @@ EXPORTS = (
     "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
     "akm_create_sparse_grid", "akm_create_dense", "akm_update_coefficients",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
-    "akm_time_apply", "akm_destroy", "akm_gmres_euler",
+    "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
 )
 
 GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
@@ -97,8 +97,9 @@ def load_library(path: str = LIB_PATH):
     L.akm_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i]
     L.akm_apply_euler.argtypes = [_vp, _d, _d, _vp, _d, _vp]
     L.akm_matrix_info.argtypes = [_vp] + [ctypes.POINTER(_i64)] * 5
-    L.akm_set_tuning.argtypes = [_vp, _i64, _i]
+    L.akm_set_tuning.argtypes = [_vp, _i, _i, _i]
     L.akm_time_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i, ctypes.POINTER(_d)]
+    L.akm_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]
     L.akm_destroy.argtypes = [_vp]
     L.akm_gmres_euler.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
                                   ctypes.POINTER(_i)]
@@ -288,8 +289,12 @@ class KronmultMatrix:
     def synchronize(self):
         _check(load_library().akm_synchronize(self._h))
 
-    def set_tuning(self, chunk_columns=0, force_generic=False):
-        _check(load_library().akm_set_tuning(self._h, int(chunk_columns), int(force_generic)))
+    def set_tuning(self, waves_per_row=0, lds_cap_bytes=-1, force_generic=False):
+        """Wavefronts per row workgroup (0 = automatic; 1, 2, 4, 8), the LDS
+        budget for the row factor tables (-1 automatic, 0 off) and the
+        block-wide LDS kernel switch."""
+        _check(load_library().akm_set_tuning(self._h, int(waves_per_row), int(lds_cap_bytes),
+                                             int(force_generic)))
 
     def apply(self, alpha, x, beta, y):
         """y = alpha*A*x + beta*y (device tensors or host arrays)."""
@@ -309,6 +314,17 @@ class KronmultMatrix:
         _check(load_library().akm_apply_euler(self._h, float(dt), float(alpha), px, float(beta),
                                               py))
         return y
+
+    def autotune(self, x, y):
+        """Times the kernel shapes on this operator (y is overwritten) and
+        keeps the fastest; returns (waves_per_row, lds_tables)."""
+        px, dx = _vec_ptr(x, self.dtype)
+        py, dy = _vec_ptr(y, self.dtype)
+        if not (dx and dy):
+            raise AkmInvalid(AKM_ERR_INVALID, "autotune needs device tensors")
+        w, l = _i(), _i()
+        _check(load_library().akm_autotune(self._h, px, py, ctypes.byref(w), ctypes.byref(l)))
+        return w.value, bool(l.value)
 
     def time_apply(self, alpha, x, beta, y, iterations):
         px, dx = _vec_ptr(x, self.dtype)

@@ -109,9 +109,10 @@ private:
 template<typename P>
 struct kron_args;
 
+//! One workgroup of `waves` wavefronts per row (rows in a.row_order).
 template<typename P>
-int launch_kronmult(int D, int n, int mode, kron_args<P> const &a, int force_generic,
-                    hipStream_t stream);
+int launch_kronmult(int D, int n, int mode, kron_args<P> const &a, int num_rows, int waves,
+                    int force_generic, hipStream_t stream);
 template<typename P>
 int launch_scale(int64_t size, P beta, P *y, P gamma, const P *xg, hipStream_t stream);
 

@@ -1,0 +1,791 @@
+// Kronmult kernel templates (included by kron_f64.hip and kron_f32.hip, one
+// translation unit per precision so the two compile in parallel).
+#pragma once
+
+#include "akm_internal.hpp"
+#include "kron_kernels.hpp"
+
+namespace akm
+{
+// ------------------------------------------------------ register stages ---
+
+//! Contraction of a leading dimension spread over the lane group: the lane
+//! with digit `digit` (stride `stride` lanes) gathers the n values of each
+//! fibre from its group partners.
+template<typename P, int n, int R, int stride>
+__device__ __forceinline__ void exchange_stage(P (&w)[R], const P *__restrict__ A, int lane,
+                                               int digit)
+{
+  P arow[n];
+#pragma unroll
+  for (int j = 0; j < n; j++)
+    arow[j] = A[j * n + digit];
+  if constexpr (n == 2 && stride == 1)
+  {
+    const P a_self = arow[digit], a_other = arow[1 - digit];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      P v = swap_pair(w[r]);
+      w[r] = fma(a_other, v, a_self * w[r]);
+    }
+  }
+  else
+  {
+    const int base = lane - digit * stride;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      P v = 0;
+#pragma unroll
+      for (int j = 0; j < n; j++)
+        v = fma(arow[j], lane_read(w[r], base + j * stride), v);
+      w[r] = v;
+    }
+  }
+}
+
+template<typename P, int D, int n, int g, int R, int K>
+__device__ __forceinline__ void exchange_stages(P (&w)[R], const P *const (&A)[D], int lane,
+                                                int sub)
+{
+  if constexpr (K >= 0)
+  {
+    constexpr int stride = ipow<n, g - 1 - K>::value;
+    const int digit = (sub / stride) % n;
+    exchange_stage<P, n, R, stride>(w, A[K], lane, digit);
+    exchange_stages<P, D, n, g, R, K - 1>(w, A, lane, sub);
+  }
+}
+
+//! One contraction stage from `src` into `dst` (which may be the same
+//! array): dst[o, s, i] = sum_j A(s, j) src[o, j, i], or dst += ... when
+//! ACCUM.  Every fibre is read before it is written, so in-place is safe.
+template<typename P, int n, int R, int inner, bool ACCUM>
+__device__ __forceinline__ void stage_into(const P (&src)[R], P (&dst)[R],
+                                           const P *__restrict__ A)
+{
+  P a[n * n];
+#pragma unroll
+  for (int q = 0; q < n * n; q++)
+    a[q] = A[q];
+  constexpr int outer = R / (n * inner);
+#pragma unroll
+  for (int o = 0; o < outer; o++)
+#pragma unroll
+    for (int i = 0; i < inner; i++)
+    {
+      P f[n];
+#pragma unroll
+      for (int j = 0; j < n; j++)
+        f[j] = src[(o * n + j) * inner + i];
+#pragma unroll
+      for (int s = 0; s < n; s++)
+      {
+        const int pos = (o * n + s) * inner + i;
+        P v = ACCUM ? fma(a[s], f[0], dst[pos]) : a[s] * f[0];
+#pragma unroll
+        for (int j = 1; j < n; j++)
+          v = fma(a[j * n + s], f[j], v);
+        dst[pos] = v;
+      }
+    }
+}
+
+//! Local stages K = D-1 .. g of one (column, term): the first reads the
+//! column tensor X (kept for the next term), the rest work in place on w,
+//! and when no exchange stage follows (g == 0) the last one accumulates
+//! straight into acc.
+template<typename P, int D, int n, int g, int R, int K, bool FROM_W>
+__device__ __forceinline__ void contract_local(const P (&X)[R], P (&w)[R], P (&acc)[R],
+                                               const P *const (&A)[D])
+{
+  if constexpr (K >= g)
+  {
+    constexpr int inner = ipow<n, D - 1 - K>::value;
+    constexpr bool first = (K == D - 1), last = (K == g), accum = last && (g == 0);
+    if constexpr (first && accum)
+    {
+      if constexpr (FROM_W)
+        stage_into<P, n, R, inner, true>(w, acc, A[K]);
+      else
+        stage_into<P, n, R, inner, true>(X, acc, A[K]);
+    }
+    else if constexpr (first && !FROM_W)
+      stage_into<P, n, R, inner, false>(X, w, A[K]);
+    else if constexpr (accum)
+      stage_into<P, n, R, inner, true>(w, acc, A[K]);
+    else
+      stage_into<P, n, R, inner, false>(w, w, A[K]);
+    contract_local<P, D, n, g, R, K - 1, FROM_W>(X, w, acc, A);
+  }
+}
+
+//! Full contraction of one (column, term) into acc; the column tensor is
+//! read from X (FROM_W = false) or from a copy already in w (FROM_W = true,
+//! which frees X for the next column's prefetch).
+template<typename P, int D, int n, int g, int R, bool FROM_W>
+__device__ __forceinline__ void contract_term(const P (&X)[R], P (&w)[R], P (&acc)[R],
+                                              const P *const (&A)[D], int lane, int sub)
+{
+  contract_local<P, D, n, g, R, D - 1, FROM_W>(X, w, acc, A);
+  if constexpr (g > 0)
+  {
+    exchange_stages<P, D, n, g, R, g - 1>(w, A, lane, sub);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      acc[r] += w[r];
+  }
+}
+
+//! Column range [beg, end) of a row.
+template<typename P, int MODE>
+__device__ __forceinline__ void row_range(kron_args<P> const &a, int row, int64_t &beg,
+                                          int64_t &end)
+{
+  if constexpr (MODE == mode_dense)
+  {
+    beg = 0;
+    end = a.cols;
+  }
+  else
+  {
+    beg = a.pntr[row];
+    end = a.pntr[row + 1];
+  }
+}
+
+// ---------------------------------------------------- register kernel ---
+
+#ifndef AKM_WAVES_PER_EU
+#define AKM_WAVES_PER_EU 1
+#endif
+
+template<typename P, int D, int n, int g, int MODE, int W>
+__global__ void __launch_bounds__(W *wave_size)
+    __attribute__((amdgpu_waves_per_eu(AKM_WAVES_PER_EU))) kron_rows(kron_args<P> a)
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int G = ipow<n, g>::value;
+  constexpr int R = N / G;
+  constexpr int CPW = wave_size / G; // columns per wavefront step
+  constexpr int LDN = N + 1;         // padded LDS row
+  constexpr int nn = n * n;
+  constexpr int EPT = (N + W * wave_size - 1) / (W * wave_size);
+  static_assert(G <= wave_size, "lane group larger than a wavefront");
+
+  __shared__ P red[CPW * LDN];
+
+  const int tid = threadIdx.x;
+  const int wave = tid / wave_size;
+  const int lane = tid - wave * wave_size;
+  const int slot = lane / G;
+  const int sub = lane - slot * G;
+  const bool lane_used = slot < CPW;
+
+  const int row = a.row_order[blockIdx.x];
+  int64_t jbeg, jend;
+  row_range<P, MODE>(a, row, jbeg, jend);
+
+  P acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    acc[r] = 0;
+
+  int r1d[D];
+  if constexpr (MODE == mode_grid)
+  {
+#pragma unroll
+    for (int k = 0; k < D; k++)
+      r1d[k] = a.elem[int64_t(row) * D + k] * a.M;
+  }
+  else if constexpr (MODE == mode_dense)
+  {
+#pragma unroll
+    for (int k = 0; k < D; k++)
+      r1d[k] = a.elem[int64_t(row + a.row_offset) * D + k];
+  }
+
+  const int64_t tstride = int64_t(D) * nn; // term stride inside a grid block
+  for (int64_t j0 = jbeg + int64_t(wave) * CPW; j0 < jend; j0 += int64_t(W) * CPW)
+  {
+    const int64_t j = j0 + slot;
+    if (lane_used && j < jend)
+    {
+      const int col = (MODE == mode_dense) ? int(j) : a.indx[j];
+      const P *__restrict__ xc = a.x + int64_t(col) * N + sub * R;
+      int64_t cbase[D];
+      if constexpr (MODE == mode_grid)
+      {
+#pragma unroll
+        for (int k = 0; k < D; k++)
+        {
+          const int off = a.off1d[r1d[k] + a.elem[int64_t(col) * D + k]];
+          cbase[k] = int64_t(off) * (tstride * a.T) + k * nn;
+        }
+      }
+      else if constexpr (MODE == mode_dense)
+      {
+        const int64_t vstride = int64_t(a.nb) * a.nb * nn;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+        {
+          const int ix = a.elem[int64_t(col + a.col_offset) * D + k];
+          cbase[k] = k * vstride + int64_t(nn) * (int64_t(ix) * a.nb + r1d[k]);
+        }
+      }
+      P X[R];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        X[r] = xc[r];
+      for (int t = 0; t < a.T; t++)
+      {
+        const P *Ak[D];
+        if constexpr (MODE == mode_grid)
+        {
+#pragma unroll
+          for (int k = 0; k < D; k++)
+            Ak[k] = a.vA + cbase[k] + t * tstride;
+        }
+        else if constexpr (MODE == mode_dense)
+        {
+          const P *tp = a.terms[t];
+#pragma unroll
+          for (int k = 0; k < D; k++)
+            Ak[k] = tp + cbase[k];
+        }
+        else
+        {
+          const int *ia = a.iA + (j * a.T + t) * D;
+#pragma unroll
+          for (int k = 0; k < D; k++)
+            Ak[k] = a.vA + ia[k];
+        }
+        P w[R];
+        contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
+      }
+    }
+  }
+
+  // sum the column slots of every wavefront, in a fixed order
+  P tot[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+    tot[i] = 0;
+  for (int wv = 0; wv < W; wv++)
+  {
+    if (wave == wv && lane_used)
+    {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        red[slot * LDN + sub * R + r] = acc[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EPT; i++)
+    {
+      const int e = tid + i * W * wave_size;
+      if (e < N)
+        for (int sl = 0; sl < CPW; sl++)
+          tot[i] += red[sl * LDN + e];
+    }
+    __syncthreads();
+  }
+  const int64_t ybase = int64_t(row) * N;
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+  {
+    const int e = tid + i * W * wave_size;
+    if (e < N)
+      store_row(a, ybase + e, tot[i]);
+  }
+}
+
+// ----------------------------------------- sparse-grid kernel, LDS tables ---
+
+constexpr int epack_bits = 10; // bits per dimension in a packed cell word
+
+//! The sparse-grid kernel with the row's factor tables in LDS.  At the start
+//! the workgroup copies, for every dimension k, the factor blocks of all 1D
+//! connections of the row's 1D cell (one contiguous range of the
+//! [dim][connection][term][nnp] array) into LDS, and builds a per-dimension
+//! table from a column's 1D index to the LDS offset of its term-0 block.
+//! The main loop then reads factors from LDS only; the next column's x slice
+//! is fetched while the current column's last term is contracted.
+template<typename P, int D, int n, int g, int W>
+__global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int G = ipow<n, g>::value;
+  constexpr int R = N / G;
+  constexpr int CPW = wave_size / G;
+  constexpr int LDN = N + 1;
+  constexpr int nnp = padded_block<P>(n);
+  constexpr int EPT = (N + W * wave_size - 1) / (W * wave_size);
+  constexpr int nthreads = W * wave_size;
+  static_assert(G <= wave_size, "lane group larger than a wavefront");
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int M = a.M;
+  int *loc = reinterpret_cast<int *>(smem_raw);
+  P *fac = reinterpret_cast<P *>(smem_raw + ((D * M * sizeof(int) + 15) / 16) * 16);
+  P *red = fac; // reused once the main loop is done
+
+  const int tid = threadIdx.x;
+  const int wave = tid / wave_size;
+  const int lane = tid - wave * wave_size;
+  const int slot = lane / G;
+  const int sub = lane - slot * G;
+  const bool lane_used = slot < CPW;
+  const int T = a.T;
+
+  const int row = a.row_order[blockIdx.x];
+  const int64_t jbeg = a.pntr[row], jend = a.pntr[row + 1];
+
+  // stage the row's factor tables and the 1D-index -> LDS-offset tables
+  int rr[D], cb[D], fb[D];
+  {
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < D; k++)
+    {
+      rr[k] = a.elem[int64_t(row) * D + k];
+      cb[k] = a.conn_pntr[rr[k]];
+      fb[k] = base;
+      base += (a.conn_pntr[rr[k] + 1] - cb[k]) * T * nnp;
+    }
+  }
+  for (int e = tid; e < D * M; e += nthreads)
+  {
+    const int k = e / M, c = e - k * M;
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < D; q++)
+      if (q == k)
+      {
+        const int off = a.off1d[rr[q] * M + c];
+        v = (off < 0) ? 0 : (off - cb[q]) * T * nnp + fb[q];
+      }
+    loc[e] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < D; k++)
+  {
+    const int len = (a.conn_pntr[rr[k] + 1] - cb[k]) * T * nnp;
+    const P *src = a.vAk + (int64_t(k) * a.nconn + cb[k]) * T * nnp;
+    for (int e = tid; e < len; e += nthreads)
+      fac[fb[k] + e] = src[e];
+  }
+  __syncthreads();
+
+  P acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    acc[r] = 0;
+
+  const int64_t jstep = int64_t(W) * CPW;
+  int64_t j = jbeg + int64_t(wave) * CPW + slot;
+  bool valid = lane_used && j < jend;
+  P X[R];
+  unsigned long long cp = 0;
+  if (valid)
+  {
+    const int col = a.indx[j];
+    cp = a.epack[col];
+    const P *xc = a.x + int64_t(col) * N + sub * R;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      X[r] = xc[r];
+  }
+  for (int64_t j0 = jbeg + int64_t(wave) * CPW; j0 < jend; j0 += jstep)
+  {
+    const int64_t jn = j + jstep;
+    const bool valid_next = lane_used && jn < jend;
+    int col_next = 0;
+    if (valid_next)
+      col_next = a.indx[jn];
+    if (valid)
+    {
+      const P *Ak[D];
+#pragma unroll
+      for (int k = 0; k < D; k++)
+      {
+        const int c1d = int((cp >> (epack_bits * k)) & ((1u << epack_bits) - 1u));
+        Ak[k] = fac + loc[k * M + c1d];
+      }
+      unsigned long long cp_next = 0;
+      P w[R];
+      for (int t = 0; t < T - 1; t++)
+      {
+        contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          Ak[k] += nnp;
+      }
+      // last term: copy the column into w, fetch the next column into X
+      // under the contraction
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        w[r] = X[r];
+      if (valid_next)
+      {
+        cp_next = a.epack[col_next];
+        const P *xn = a.x + int64_t(col_next) * N + sub * R;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          X[r] = xn[r];
+      }
+      contract_term<P, D, n, g, R, true>(X, w, acc, Ak, lane, sub);
+      cp = cp_next;
+    }
+    j = jn;
+    valid = valid_next;
+  }
+  __syncthreads(); // factor tables are dead from here on
+
+  P tot[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+    tot[i] = 0;
+  for (int wv = 0; wv < W; wv++)
+  {
+    if (wave == wv && lane_used)
+    {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        red[slot * LDN + sub * R + r] = acc[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EPT; i++)
+    {
+      const int e = tid + i * nthreads;
+      if (e < N)
+        for (int sl = 0; sl < CPW; sl++)
+          tot[i] += red[sl * LDN + e];
+    }
+    __syncthreads();
+  }
+  const int64_t ybase = int64_t(row) * N;
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+  {
+    const int e = tid + i * nthreads;
+    if (e < N)
+      store_row(a, ybase + e, tot[i]);
+  }
+}
+
+// ------------------------------------------------------- generic kernel ---
+
+constexpr int generic_threads = 256;
+constexpr int generic_max_ept = 16; // N <= 4096
+
+//! Block-wide kernel for any (d, n) with n^d <= 4096: the tensor of one
+//! (column, term) lives in LDS (one buffer, updated in place stage by
+//! stage), each thread owns the entries it writes.  One block per row.
+template<typename P, int MODE>
+__global__ void __launch_bounds__(generic_threads) kron_generic(kron_args<P> a, int D, int n)
+{
+  extern __shared__ unsigned char smem_raw[];
+  int N = 1;
+  for (int k = 0; k < D; k++)
+    N *= n;
+  P *buf = reinterpret_cast<P *>(smem_raw);
+  const int tid = threadIdx.x;
+  const int nn = n * n;
+
+  const int row = a.row_order[blockIdx.x];
+  int64_t jbeg, jend;
+  row_range<P, MODE>(a, row, jbeg, jend);
+
+  int st[6];
+  {
+    int s = 1;
+    for (int k = D - 1; k >= 0; k--)
+    {
+      st[k] = s;
+      s *= n;
+    }
+  }
+  int r1d[6] = {0, 0, 0, 0, 0, 0};
+  if (MODE == mode_grid)
+    for (int k = 0; k < D; k++)
+      r1d[k] = a.elem[int64_t(row) * D + k] * a.M;
+  else if (MODE == mode_dense)
+    for (int k = 0; k < D; k++)
+      r1d[k] = a.elem[int64_t(row + a.row_offset) * D + k];
+
+  P acc[generic_max_ept];
+  for (int i = 0; i < generic_max_ept; i++)
+    acc[i] = 0;
+
+  const int64_t tstride = int64_t(D) * nn;
+  for (int64_t j = jbeg; j < jend; j++)
+  {
+    const int col = (MODE == mode_dense) ? int(j) : a.indx[j];
+    const P *xc = a.x + int64_t(col) * N;
+    int64_t cbase[6];
+    if (MODE == mode_grid)
+      for (int k = 0; k < D; k++)
+        cbase[k] = int64_t(a.off1d[r1d[k] + a.elem[int64_t(col) * D + k]]) * (tstride * a.T) +
+                   k * nn;
+    else if (MODE == mode_dense)
+      for (int k = 0; k < D; k++)
+        cbase[k] = k * (int64_t(a.nb) * a.nb * nn) +
+                   int64_t(nn) * (int64_t(a.elem[int64_t(col + a.col_offset) * D + k]) * a.nb +
+                                  r1d[k]);
+    for (int t = 0; t < a.T; t++)
+    {
+      __syncthreads();
+      for (int e = tid; e < N; e += generic_threads)
+        buf[e] = xc[e];
+      __syncthreads();
+      for (int k = 0; k < D; k++)
+      {
+        const P *A;
+        if (MODE == mode_grid)
+          A = a.vA + cbase[k] + t * tstride;
+        else if (MODE == mode_dense)
+          A = a.terms[t] + cbase[k];
+        else
+          A = a.vA + a.iA[(j * a.T + t) * D + k];
+        // every thread computes its entries into registers, then the
+        // stage result replaces the tensor in place
+        P v[generic_max_ept];
+#pragma unroll
+        for (int i = 0; i < generic_max_ept; i++)
+        {
+          const int e = tid + i * generic_threads;
+          v[i] = 0;
+          if (e < N)
+          {
+            const int s = (e / st[k]) % n;
+            const int base = e - s * st[k];
+            for (int q = 0; q < n; q++)
+              v[i] = fma(A[q * n + s], buf[base + q * st[k]], v[i]);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < generic_max_ept; i++)
+        {
+          const int e = tid + i * generic_threads;
+          if (e < N)
+            buf[e] = v[i];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < generic_max_ept; i++)
+      {
+        const int e = tid + i * generic_threads;
+        if (e < N)
+          acc[i] += buf[e];
+      }
+    }
+  }
+  const int64_t ybase = int64_t(row) * N;
+#pragma unroll
+  for (int i = 0; i < generic_max_ept; i++)
+  {
+    const int e = tid + i * generic_threads;
+    if (e < N)
+      store_row(a, ybase + e, acc[i]);
+  }
+}
+
+// --------------------------------------------------------- scale kernel ---
+
+//! y = beta*y + gamma*xg (operator with no terms, or alpha == 0).
+template<typename P>
+__global__ void __launch_bounds__(256) kron_scale(int64_t size, P beta, P *y, P gamma,
+                                                  const P *xg)
+{
+  for (int64_t i = blockIdx.x * int64_t(256) + threadIdx.x; i < size;
+       i += int64_t(gridDim.x) * 256)
+  {
+    P v = (beta == P{0}) ? P{0} : beta * y[i];
+    if (gamma != P{0})
+      v += gamma * xg[i];
+    y[i] = v;
+  }
+}
+
+// ------------------------------------------------------------- dispatch ---
+
+//! Number of leading dimensions spread over the lane group: the smallest g
+//! with n^(d-g) <= the register budget (32 FP64 / 64 FP32 entries per lane).
+#ifndef AKM_F64_REG_ENTRIES
+#define AKM_F64_REG_ENTRIES 32
+#endif
+
+template<typename P>
+constexpr int lead_dims(int D, int n)
+{
+  const int budget = (sizeof(P) == 8) ? AKM_F64_REG_ENTRIES : 64;
+  int g = 0, R = 1;
+  for (int k = 0; k < D; k++)
+    R *= n;
+  while (R > budget && g < D)
+  {
+    R /= n;
+    g++;
+  }
+  return g;
+}
+
+template<typename P, int D, int n, int MODE, int W>
+bool launch_rows_w(kron_args<P> const &a, int num_rows, hipStream_t stream)
+{
+  constexpr int g = lead_dims<P>(D, n);
+  if constexpr (ipow<n, g>::value > wave_size)
+  {
+    return false;
+  }
+  else
+  {
+    kron_rows<P, D, n, g, MODE, W><<<num_rows, W * wave_size, 0, stream>>>(a);
+    return true;
+  }
+}
+
+//! LDS bytes of kron_grid_lds for the matrix, or 0 if it does not fit.
+template<typename P, int D, int n, int g, int W>
+size_t grid_lds_bytes(kron_args<P> const &a)
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int CPW = wave_size / ipow<n, g>::value;
+  const size_t locb = ((size_t(D) * a.M * sizeof(int) + 15) / 16) * 16;
+  const size_t tab = size_t(a.max_table_elems) * sizeof(P);
+  const size_t redb = size_t(CPW) * (N + 1) * sizeof(P);
+  const size_t total = locb + (tab > redb ? tab : redb);
+  return (a.vAk != nullptr && total <= size_t(a.lds_cap)) ? total : 0;
+}
+
+template<typename P, int D, int n, int W>
+bool launch_grid_lds_w(kron_args<P> const &a, int num_rows, hipStream_t stream)
+{
+  constexpr int g = lead_dims<P>(D, n);
+  if constexpr (ipow<n, g>::value > wave_size)
+  {
+    return false;
+  }
+  else
+  {
+    const size_t lds = grid_lds_bytes<P, D, n, g, W>(a);
+    if (lds == 0)
+      return false;
+    auto kern = kron_grid_lds<P, D, n, g, W>;
+    if (lds > 64 * 1024)
+    {
+      if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds)) != hipSuccess)
+      {
+        (void)hipGetLastError();
+        return false;
+      }
+    }
+    kern<<<num_rows, W * wave_size, lds, stream>>>(a);
+    return true;
+  }
+}
+
+template<typename P, int D, int n, int MODE>
+bool launch_rows(kron_args<P> const &a, int num_rows, int waves, hipStream_t stream)
+{
+  if constexpr (MODE == mode_grid)
+  {
+    if (a.lds_cap > 0)
+    {
+      bool done = false;
+      switch (waves)
+      {
+      case 1: done = launch_grid_lds_w<P, D, n, 1>(a, num_rows, stream); break;
+      case 2: done = launch_grid_lds_w<P, D, n, 2>(a, num_rows, stream); break;
+      case 8: done = launch_grid_lds_w<P, D, n, 8>(a, num_rows, stream); break;
+      default: done = launch_grid_lds_w<P, D, n, 4>(a, num_rows, stream); break;
+      }
+      if (done)
+        return true;
+    }
+    // the performance path carries every workgroup size the tuner may pick
+    switch (waves)
+    {
+    case 1: return launch_rows_w<P, D, n, MODE, 1>(a, num_rows, stream);
+    case 2: return launch_rows_w<P, D, n, MODE, 2>(a, num_rows, stream);
+    case 8: return launch_rows_w<P, D, n, MODE, 8>(a, num_rows, stream);
+    default: return launch_rows_w<P, D, n, MODE, 4>(a, num_rows, stream);
+    }
+  }
+  else
+  {
+    return launch_rows_w<P, D, n, MODE, 4>(a, num_rows, stream);
+  }
+}
+
+template<typename P, int MODE>
+bool dispatch_rows(int D, int n, kron_args<P> const &a, int num_rows, int waves,
+                   hipStream_t stream)
+{
+#define AKM_CASE(DD, NN)                                                                           \
+  if (D == DD && n == NN)                                                                          \
+    return launch_rows<P, DD, NN, MODE>(a, num_rows, waves, stream);
+  AKM_CASE(1, 1) AKM_CASE(1, 2) AKM_CASE(1, 3) AKM_CASE(1, 4) AKM_CASE(1, 5)
+  AKM_CASE(2, 1) AKM_CASE(2, 2) AKM_CASE(2, 3) AKM_CASE(2, 4) AKM_CASE(2, 5)
+  AKM_CASE(3, 1) AKM_CASE(3, 2) AKM_CASE(3, 3) AKM_CASE(3, 4) AKM_CASE(3, 5)
+  AKM_CASE(4, 1) AKM_CASE(4, 2) AKM_CASE(4, 3) AKM_CASE(4, 4)
+  AKM_CASE(5, 1) AKM_CASE(5, 2) AKM_CASE(5, 3)
+  AKM_CASE(6, 1) AKM_CASE(6, 2) AKM_CASE(6, 3)
+#undef AKM_CASE
+  return false;
+}
+
+template<typename P>
+int launch_kronmult_impl(int D, int n, int mode, kron_args<P> const &a, int num_rows,
+                         int waves, int force_generic, hipStream_t stream)
+{
+  if (num_rows == 0)
+    return AKM_SUCCESS;
+  int N = 1;
+  for (int k = 0; k < D; k++)
+    N *= n;
+  bool done = false;
+  if (!force_generic)
+  {
+    if (mode == mode_ia)
+      done = dispatch_rows<P, mode_ia>(D, n, a, num_rows, waves, stream);
+    else if (mode == mode_grid)
+      done = dispatch_rows<P, mode_grid>(D, n, a, num_rows, waves, stream);
+    else
+      done = dispatch_rows<P, mode_dense>(D, n, a, num_rows, waves, stream);
+  }
+  if (!done)
+  {
+    if (N > generic_threads * generic_max_ept)
+      return set_error(AKM_ERR_UNSUPPORTED, "kronmult: tensor size n^d = %d exceeds 4096", N);
+    const size_t lds = sizeof(P) * size_t(N);
+    if (mode == mode_ia)
+      kron_generic<P, mode_ia><<<num_rows, generic_threads, lds, stream>>>(a, D, n);
+    else if (mode == mode_grid)
+      kron_generic<P, mode_grid><<<num_rows, generic_threads, lds, stream>>>(a, D, n);
+    else
+      kron_generic<P, mode_dense><<<num_rows, generic_threads, lds, stream>>>(a, D, n);
+  }
+  return check_launch("kronmult");
+}
+
+template<typename P>
+int launch_scale_impl(int64_t size, P beta, P *y, P gamma, const P *xg, hipStream_t stream)
+{
+  if (size == 0)
+    return AKM_SUCCESS;
+  int64_t blocks = (size + 255) / 256;
+  if (blocks > 4096)
+    blocks = 4096;
+  kron_scale<P><<<int(blocks), 256, 0, stream>>>(size, beta, y, gamma, xg);
+  return check_launch("kronmult scale");
+}
+
+} // namespace akm

@@ -7,16 +7,17 @@
 // (src/device/asgard_spkronmult_cpu.cpp:84-433).  The gamma term fuses the
 // identity of the implicit operator (I - dt A) into the apply.
 //
-// Work decomposition: the CSR column list of every row is cut into chunks;
-// one wavefront owns one chunk.  Inside a wavefront, G = n^g lanes share one
-// column: lane `sub` of the group holds the R = n^(d-g) tensor entries whose
-// leading g digits equal `sub`.  Contraction stages over the trailing d-g
-// dimensions run in place in registers; stages over the leading g dimensions
-// exchange values inside the lane group (DPP for pairs, ds_bpermute otherwise).
-// The per-lane accumulators are summed over the column slots through LDS in a
-// fixed order, and rows cut into several chunks are finished by the last
-// chunk to arrive, which sums the chunk partials in chunk order.  No atomics
-// touch y, so results are deterministic run to run.
+// Work decomposition: one workgroup of W wavefronts owns one output row (a
+// cell); rows are dispatched longest first.  The W wavefronts take turns over
+// the row's column list, CPW columns per wavefront step.  Inside a
+// wavefront, G = n^g lanes share one column: lane `sub` of the group holds
+// the R = n^(d-g) tensor entries whose leading g digits equal `sub`.
+// Contraction stages over the trailing d-g dimensions run in place in
+// registers; stages over the leading g dimensions exchange values inside the
+// lane group (DPP for pairs, ds_bpermute otherwise).  At the end the
+// per-lane accumulators are summed through LDS in a fixed order (wavefront by
+// wavefront, slot by slot), and the row is stored once.  No atomics, no
+// cross-workgroup traffic: results are deterministic run to run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,25 +53,29 @@ template<typename P>
 struct kron_args
 {
   // factor data
-  const P *vA;          // mode_ia / mode_grid: factor blocks
-  const int *iA;        // mode_ia: (pair*T + t)*d + k -> offset into vA
-  const int *indx;      // mode_ia / mode_grid: column cell ids (CSR)
-  const int *elem;      // mode_grid / mode_dense: 1D index per (cell, dim)
-  const int *off1d;     // mode_grid: connect_1d position, [M*M], -1 if none
-  int M;                // mode_grid: 2^max_level
+  const P *vA;           // mode_ia / mode_grid: factor blocks
+  const int *iA;         // mode_ia: (pair*T + t)*d + k -> offset into vA
+  const int64_t *pntr;   // mode_ia / mode_grid: CSR row pointer
+  const int *indx;       // mode_ia / mode_grid: column cell ids (CSR)
+  const int *elem;       // mode_grid / mode_dense: 1D index per (cell, dim)
+  const int *off1d;      // mode_grid: connect_1d position, [M*M], -1 if none
+  int M;                 // mode_grid: 2^max_level
   const P *const *terms; // mode_dense: per-term block arrays (device)
-  int nb;               // mode_dense: num_1d_blocks
-  int row_offset;       // mode_dense
-  int col_offset;       // mode_dense
-  int T;                // number of terms
-  // work decomposition
-  int num_chunks;
-  const int *chunk_row;
-  const int64_t *chunk_beg;
-  const int64_t *chunk_end;
-  const int *row_chunk_begin; // rows + 1
-  int *counters;              // one per row, zero between launches
-  P *partial;                 // num_chunks * n^d
+  int nb;                // mode_dense: num_1d_blocks
+  int row_offset;        // mode_dense
+  int col_offset;        // mode_dense
+  int cols;              // mode_dense: number of column cells
+  int T;                 // number of terms
+  // mode_grid with LDS factor tables
+  const P *vAk;                     // factors as [dim][connection][term][nnp]
+  const int *conn_pntr;             // connect_1d row pointer (M + 1)
+  const unsigned long long *epack;  // 1D indices of a cell, 10 bits per dim
+  int nconn;                        // connect_1d connections
+  int nnp;                          // padded n*n block size in vAk
+  int64_t max_table_elems;          // largest per-row factor table (elements)
+  int lds_cap;                      // LDS bytes a workgroup may use (0: no tables)
+  // rows in dispatch order (longest first)
+  const int *row_order;
   // vectors
   const P *x;
   P *y;
@@ -78,14 +83,11 @@ struct kron_args
   P alpha, beta, gamma;
 };
 
-//! Maps a block index to a chunk so that each of the 8 XCDs (blocks are
-//! dealt to XCDs round-robin) works on one contiguous range of chunks.
-__device__ __forceinline__ int xcd_chunk(int b, int num)
+//! Padded n*n block size of the re-laid-out factor array (16-byte rows).
+template<typename P>
+__host__ __device__ constexpr int padded_block(int n)
 {
-  constexpr int num_xcd = 8;
-  int x = b % num_xcd, i = b / num_xcd;
-  int q = num / num_xcd, rem = num % num_xcd;
-  return x * q + min(x, rem) + i;
+  return (sizeof(P) == 8) ? (n * n + 1) / 2 * 2 : (n * n + 3) / 4 * 4;
 }
 
 __device__ __forceinline__ double swap_pair(double v)

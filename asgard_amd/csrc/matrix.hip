@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 namespace akm
@@ -41,86 +43,77 @@ static int ipow_int(int n, int d)
   return r;
 }
 
-int lanes_per_column(akm_prec prec, int d, int n, int force_generic)
+int waves_per_row(akm_matrix_s const &m)
 {
-  // mirrors lead_dims() + the register-kernel table of kron_kernels.hip;
-  // 0 means the block-wide LDS kernel
-  static const int reg_cases[][2] = {{1, 1}, {1, 2}, {1, 3}, {1, 4}, {1, 5}, {2, 1}, {2, 2},
-                                     {2, 3}, {2, 4}, {2, 5}, {3, 1}, {3, 2}, {3, 3}, {3, 4},
-                                     {3, 5}, {4, 1}, {4, 2}, {4, 3}, {4, 4}, {5, 1}, {5, 2},
-                                     {5, 3}, {6, 1}, {6, 2}, {6, 3}};
-  if (force_generic)
-    return 0;
-  bool listed = false;
-  for (auto const &c : reg_cases)
-    if (c[0] == d && c[1] == n)
-      listed = true;
-  if (!listed)
-    return 0;
-  const int budget = (prec == AKM_F64) ? 32 : 64;
-  int R = ipow_int(n, d), G = 1;
-  while (R > budget)
-  {
-    R /= n;
-    G *= n;
-  }
-  return (G <= wave_size) ? G : 0;
+  if (m.waves > 0)
+    return m.waves;
+  // enough wavefronts to cover the 1024 SIMDs several times over
+  const int64_t target = 8 * 1024;
+  int w = 1;
+  while (w < 8 && int64_t(m.rows) * w < target)
+    w *= 2;
+  return w;
 }
 
-//! Cuts every row's column range into chunks of about `chunk_columns`
-//! columns (a multiple of the columns a wavefront processes per step).
-static int build_chunks(akm_matrix_s &m)
+//! Row dispatch order: longest column list first (stable), so the heaviest
+//! workgroups start first and the tail of the launch is short.
+static int build_schedule(akm_matrix_s &m)
 {
-  const int G = lanes_per_column(m.prec, m.d, m.n, m.force_generic);
-  const int64_t cpw = (G > 0) ? wave_size / G : 1;
-  int64_t cc = m.chunk_columns;
-  if (cc <= 0)
-  {
-    // about 16 wavefronts per SIMD (1024 SIMDs), 1 to 64 steps per chunk
-    const int64_t target = 16 * 1024;
-    int64_t steps = (m.nnz + cpw * target - 1) / (cpw * target);
-    steps = std::max<int64_t>(1, std::min<int64_t>(steps, 64));
-    if (G == 0)
-      steps = std::max<int64_t>(1, std::min<int64_t>(m.nnz / (4 * 1024), 32));
-    cc = cpw * steps;
-  }
-  cc = std::max<int64_t>(cpw, (cc + cpw - 1) / cpw * cpw);
-
-  std::vector<int> crow, rbeg(m.rows + 1, 0);
-  std::vector<int64_t> cb, ce;
+  std::vector<int> order(m.rows);
   for (int r = 0; r < m.rows; r++)
+    order[r] = r;
+  if (m.mode != mode_dense)
   {
-    const int64_t b = (m.mode == mode_dense) ? 0 : m.pntr[r];
-    const int64_t e = (m.mode == mode_dense) ? m.cols : m.pntr[r + 1];
-    rbeg[r] = static_cast<int>(crow.size());
-    const int64_t len = e - b;
-    const int64_t nch = std::max<int64_t>(1, (len + cc - 1) / cc);
-    int64_t piece = (len + nch - 1) / nch;
-    piece = std::max<int64_t>(cpw, (piece + cpw - 1) / cpw * cpw);
-    int64_t s = b;
-    do
-    {
-      const int64_t f = std::min(e, s + piece);
-      crow.push_back(r);
-      cb.push_back(s);
-      ce.push_back(f);
-      s = f;
-    } while (s < e);
-    if (static_cast<int64_t>(crow.size()) > (int64_t(1) << 31) - 1)
-      return set_error(AKM_ERR_INVALID, "kronmult: too many work chunks");
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (m.pntr[a + 1] - m.pntr[a]) > (m.pntr[b + 1] - m.pntr[b]);
+    });
+    AKM_TRY(m.pntr_dev.upload(m.pntr.data(), int64_t(m.pntr.size()), m.stream));
   }
-  rbeg[m.rows] = static_cast<int>(crow.size());
-  m.num_chunks = static_cast<int>(crow.size());
-  AKM_TRY(m.chunk_row.upload(crow.data(), crow.size(), m.stream));
-  AKM_TRY(m.chunk_beg.upload(cb.data(), cb.size(), m.stream));
-  AKM_TRY(m.chunk_end.upload(ce.data(), ce.size(), m.stream));
-  AKM_TRY(m.row_chunk_begin.upload(rbeg.data(), rbeg.size(), m.stream));
-  AKM_TRY(m.counters.allocate(m.rows));
-  AKM_HIP(hipMemsetAsync(m.counters.data(), 0, sizeof(int) * m.rows, m.stream));
-  AKM_TRY(m.partial.allocate(int64_t(m.num_chunks) * m.N * int64_t(m.esize)));
-  AKM_HIP(hipStreamSynchronize(m.stream));
-  m.chunks_ready = true;
+  AKM_TRY(m.row_order.upload(order.data(), int64_t(order.size()), m.stream));
   return AKM_SUCCESS;
+}
+
+//! Re-lays the reference's vA ([connection][term][dim][n*n]) out as
+//! [dim][connection][term][nnp] for the LDS-table kernel: the blocks one
+//! row's 1D cell can touch in one dimension become one contiguous range.
+template<typename P>
+static void relayout_vA(akm_matrix_s const &m, const P *vA, std::vector<P> &out)
+{
+  const int d = m.d, T = m.T, nn = m.n * m.n, nnp = m.nnp;
+  out.assign(size_t(d) * m.nconn * T * nnp, P{0});
+  for (int64_t j = 0; j < m.nconn; j++)
+    for (int t = 0; t < T; t++)
+      for (int k = 0; k < d; k++)
+      {
+        const P *src = vA + ((j * T + t) * d + k) * nn;
+        P *dst = out.data() + ((int64_t(k) * m.nconn + j) * T + t) * nnp;
+        for (int q = 0; q < nn; q++)
+          dst[q] = src[q];
+      }
+}
+
+static int upload_vAk(akm_matrix_s &m, const void *vA)
+{
+  if (m.prec == AKM_F64)
+  {
+    std::vector<double> v;
+    relayout_vA(m, static_cast<const double *>(vA), v);
+    return m.vAk.upload(reinterpret_cast<const unsigned char *>(v.data()),
+                        int64_t(v.size() * sizeof(double)), m.stream);
+  }
+  std::vector<float> v;
+  relayout_vA(m, static_cast<const float *>(vA), v);
+  return m.vAk.upload(reinterpret_cast<const unsigned char *>(v.data()),
+                      int64_t(v.size() * sizeof(float)), m.stream);
+}
+
+//! LDS bytes the sparse-grid kernel may use: the whole 160 KB of a CU minus
+//! a margin unless the caller set a cap.
+static int lds_cap_bytes(akm_matrix_s const &m)
+{
+  if (m.lds_cap >= 0)
+    return m.lds_cap;
+  return 160 * 1024 - 4096;
 }
 
 template<typename P>
@@ -129,11 +122,10 @@ static int apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gam
 {
   if (m.T == 0 || alpha == P{0} || m.rows == 0)
     return launch_scale<P>(m.output_size(), beta, y, gamma, xg, m.stream);
-  if (!m.chunks_ready)
-    AKM_TRY(build_chunks(m));
   kron_args<P> a{};
   a.vA = reinterpret_cast<const P *>(m.vA.data());
   a.iA = m.iA.data();
+  a.pntr = m.pntr_dev.data();
   a.indx = m.indx.data();
   a.elem = m.elem.data();
   a.off1d = m.off1d.data();
@@ -142,21 +134,24 @@ static int apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gam
   a.nb = m.nb;
   a.row_offset = m.row_offset;
   a.col_offset = m.col_offset;
+  a.cols = m.cols;
   a.T = m.T;
-  a.num_chunks = m.num_chunks;
-  a.chunk_row = m.chunk_row.data();
-  a.chunk_beg = m.chunk_beg.data();
-  a.chunk_end = m.chunk_end.data();
-  a.row_chunk_begin = m.row_chunk_begin.data();
-  a.counters = m.counters.data();
-  a.partial = reinterpret_cast<P *>(m.partial.data());
+  a.vAk = reinterpret_cast<const P *>(m.vAk.data());
+  a.conn_pntr = m.conn_pntr.data();
+  a.epack = m.epack.data();
+  a.nconn = m.nconn;
+  a.nnp = m.nnp;
+  a.max_table_elems = m.max_table_elems;
+  a.lds_cap = (m.vAk.data() != nullptr) ? lds_cap_bytes(m) : 0;
+  a.row_order = m.row_order.data();
   a.x = x;
   a.y = y;
   a.xg = xg;
   a.alpha = alpha;
   a.beta = beta;
   a.gamma = gamma;
-  return launch_kronmult<P>(m.d, m.n, m.mode, a, m.force_generic, m.stream);
+  return launch_kronmult<P>(m.d, m.n, m.mode, a, m.rows, waves_per_row(m), m.force_generic,
+                            m.stream);
 }
 
 int apply_device(akm_matrix_s &m, double alpha, const void *x, double beta, void *y,
@@ -314,7 +309,7 @@ akm_status akm_create_sparse(int d, int n, int rows, int cols, int T, const int6
   AKM_TRY(m->iA.upload(iA, nnz * T * d));
   m->num_vA = num_vA;
   AKM_TRY(m->vA.upload(static_cast<const unsigned char *>(vA), num_vA * m->esize));
-  AKM_TRY(build_chunks(*m));
+  AKM_TRY(build_schedule(*m));
   *mat = m.release();
   return AKM_SUCCESS;
 }
@@ -360,10 +355,44 @@ akm_status akm_create_sparse_grid(int d, int n, int64_t num_cells, const int *ta
   AKM_TRY(m->elem.upload(elem.data(), int64_t(elem.size())));
   AKM_TRY(m->off1d.upload(offtab.data(), int64_t(offtab.size())));
   m->M = 1 << max_level;
+  m->max_level = max_level;
   m->num_vA = num_vA;
+  m->nconn = static_cast<int>(nconn);
+  m->conn_pntr_host = p1;
   if (T > 0)
+  {
     AKM_TRY(m->vA.upload(static_cast<const unsigned char *>(vA), num_vA * m->esize));
-  AKM_TRY(build_chunks(*m));
+    if (max_level <= 10)
+    {
+      // LDS-table kernel inputs: packed cell words, connect_1d row pointer,
+      // factors re-laid out per dimension, largest per-row table
+      std::vector<unsigned long long> pack(num_cells);
+      for (int64_t c = 0; c < num_cells; c++)
+      {
+        unsigned long long w = 0;
+        for (int k = 0; k < d; k++)
+          w |= static_cast<unsigned long long>(elem[c * d + k]) << (10 * k);
+        pack[c] = w;
+      }
+      AKM_TRY(m->epack.upload(pack.data(), num_cells));
+      AKM_TRY(m->conn_pntr.upload(p1.data(), int64_t(p1.size())));
+      m->nnp = (m->prec == AKM_F64) ? padded_block<double>(n) : padded_block<float>(n);
+      AKM_TRY(upload_vAk(*m, vA));
+      int64_t mx = 0;
+      for (int64_t c = 0; c < num_cells; c++)
+      {
+        int64_t s = 0;
+        for (int k = 0; k < d; k++)
+        {
+          const int r = elem[c * d + k];
+          s += int64_t(p1[r + 1] - p1[r]) * T * m->nnp;
+        }
+        mx = std::max(mx, s);
+      }
+      m->max_table_elems = mx;
+    }
+  }
+  AKM_TRY(build_schedule(*m));
   *mat = m.release();
   return AKM_SUCCESS;
 }
@@ -398,7 +427,7 @@ akm_status akm_create_dense(int d, int n, int rows, int cols, int T, const void 
   }
   if (T > 0)
     AKM_TRY(m->term_ptrs.upload(ptrs.data(), T));
-  AKM_TRY(build_chunks(*m));
+  AKM_TRY(build_schedule(*m));
   *mat = m.release();
   return AKM_SUCCESS;
 }
@@ -421,6 +450,8 @@ akm_status akm_update_coefficients(akm_matrix mat, const void *values)
   {
     AKM_HIP(hipMemcpyAsync(mat->vA.data(), values, mat->num_vA * mat->esize,
                            hipMemcpyHostToDevice, mat->stream));
+    if (mat->vAk.data() != nullptr)
+      AKM_TRY(upload_vAk(*mat, values));
   }
   AKM_HIP(hipStreamSynchronize(mat->stream));
   return AKM_SUCCESS;
@@ -472,15 +503,21 @@ akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_
   return AKM_SUCCESS;
 }
 
-akm_status akm_set_tuning(akm_matrix mat, int64_t chunk_columns, int force_generic)
+akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
+                          int force_generic)
 {
   if (mat == nullptr)
     return set_error(AKM_ERR_INVALID, "null matrix");
-  AKM_HIP(hipStreamSynchronize(mat->stream));
-  mat->chunk_columns = chunk_columns;
+  if (waves_per_row != 0 && waves_per_row != 1 && waves_per_row != 2 && waves_per_row != 4 &&
+      waves_per_row != 8)
+    return set_error(AKM_ERR_INVALID, "waves per row must be 0, 1, 2, 4 or 8 (got %d)",
+                     waves_per_row);
+  if (lds_cap_bytes > 160 * 1024)
+    return set_error(AKM_ERR_INVALID, "LDS cap %d above the 160 KB of a CU", lds_cap_bytes);
+  mat->waves = waves_per_row;
+  mat->lds_cap = (lds_cap_bytes < 0) ? -1 : lds_cap_bytes;
   mat->force_generic = force_generic;
-  mat->chunks_ready = false;
-  return build_chunks(*mat);
+  return AKM_SUCCESS;
 }
 
 akm_status akm_time_apply(akm_matrix mat, double alpha, const void *x, double beta, void *y,
@@ -488,7 +525,7 @@ akm_status akm_time_apply(akm_matrix mat, double alpha, const void *x, double be
 {
   if (mat == nullptr || ms_per_apply == nullptr || iterations <= 0)
     return set_error(AKM_ERR_INVALID, "time_apply: invalid argument");
-  AKM_TRY(apply_device(*mat, alpha, x, beta, y, 0.0, nullptr)); // builds chunks
+  AKM_TRY(apply_device(*mat, alpha, x, beta, y, 0.0, nullptr)); // warm-up
   hipEvent_t start, stop;
   AKM_HIP(hipEventCreate(&start));
   AKM_HIP(hipEventCreate(&stop));
@@ -502,6 +539,65 @@ akm_status akm_time_apply(akm_matrix mat, double alpha, const void *x, double be
   AKM_HIP(hipEventDestroy(start));
   AKM_HIP(hipEventDestroy(stop));
   *ms_per_apply = double(ms) / iterations;
+  return AKM_SUCCESS;
+}
+
+akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_row,
+                        int *lds_tables)
+{
+  if (mat == nullptr || x == nullptr || y == nullptr)
+    return set_error(AKM_ERR_INVALID, "autotune: null argument");
+  struct choice
+  {
+    int waves, lds;
+  };
+  static std::mutex mtx;
+  static std::map<std::vector<int64_t>, choice> cache;
+  const std::vector<int64_t> key = {int64_t(mat->prec), mat->d,      mat->n,  mat->T,
+                                    mat->max_level,     mat->rows,   mat->nnz, int64_t(mat->mode)};
+  choice best{0, -1};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lock(mtx);
+    auto it = cache.find(key);
+    if (it != cache.end())
+    {
+      best = it->second;
+      cached = true;
+    }
+  }
+  if (!cached)
+  {
+    // enough applies per candidate for ~2 ms of timing, at most 50
+    double probe = 0;
+    mat->waves = 4;
+    mat->lds_cap = -1;
+    AKM_TRY(akm_time_apply(mat, 1.0, x, 0.0, y, 1, &probe));
+    const int iters = std::max(1, std::min(50, int(2.0 / std::max(probe, 1e-3))));
+    double best_ms = 1e300;
+    const int lds_opts[2] = {-1, 0};
+    for (int li = 0; li < (mat->vAk.data() != nullptr ? 2 : 1); li++)
+      for (int w : {1, 2, 4, 8})
+      {
+        mat->waves = w;
+        mat->lds_cap = (mat->vAk.data() != nullptr) ? lds_opts[li] : 0;
+        double ms = 0;
+        AKM_TRY(akm_time_apply(mat, 1.0, x, 0.0, y, iters, &ms));
+        if (ms < best_ms)
+        {
+          best_ms = ms;
+          best = {w, mat->lds_cap};
+        }
+      }
+    std::lock_guard<std::mutex> lock(mtx);
+    cache[key] = best;
+  }
+  mat->waves = best.waves;
+  mat->lds_cap = best.lds;
+  if (waves_per_row)
+    *waves_per_row = best.waves;
+  if (lds_tables)
+    *lds_tables = (best.lds != 0 && mat->vAk.data() != nullptr) ? 1 : 0;
   return AKM_SUCCESS;
 }
 

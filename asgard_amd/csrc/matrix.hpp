@@ -26,19 +26,26 @@ struct akm_matrix_s
   // grid mode
   akm::dev_buffer<int> elem, off1d;
   int M = 0;
+  int max_level = 0;
+  // grid mode, LDS factor tables: factors as [dim][connection][term][nnp],
+  // connect_1d row pointer, packed cell 1D indices
+  akm::dev_buffer<unsigned char> vAk;
+  akm::dev_buffer<int> conn_pntr;
+  akm::dev_buffer<unsigned long long> epack;
+  std::vector<int> conn_pntr_host;
+  int nconn = 0, nnp = 0;
+  int64_t max_table_elems = 0;
+  int lds_cap = -1; // -1 automatic, 0 disables the LDS tables
   // dense mode
   std::vector<akm::dev_buffer<unsigned char>> terms;
   akm::dev_buffer<void *> term_ptrs;
   int nb = 0, row_offset = 0, col_offset = 0;
 
-  // work decomposition
-  int64_t chunk_columns = 0; // 0 = automatic
+  // work decomposition: one workgroup per row, rows longest first
+  int waves = 0;        // wavefronts per row workgroup (0 = automatic)
   int force_generic = 0;
-  bool chunks_ready = false;
-  int num_chunks = 0;
-  akm::dev_buffer<int> chunk_row, row_chunk_begin, counters;
-  akm::dev_buffer<int64_t> chunk_beg, chunk_end;
-  akm::dev_buffer<unsigned char> partial;
+  akm::dev_buffer<int64_t> pntr_dev;
+  akm::dev_buffer<int> row_order;
 
   // workspaces for host-resident vectors
   akm::dev_buffer<unsigned char> xws, yws;
@@ -59,6 +66,6 @@ namespace akm
 int apply_device(akm_matrix_s &m, double alpha, const void *x, double beta, void *y,
                  double gamma, const void *xg);
 
-//! Columns per chunk and lanes per column of the kernel the matrix uses.
-int lanes_per_column(akm_prec prec, int d, int n, int force_generic);
+//! Wavefronts per row workgroup used for the matrix (explicit or automatic).
+int waves_per_row(akm_matrix_s const &m);
 } // namespace akm

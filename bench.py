@@ -67,13 +67,32 @@ def cpu_baseline(d, n, table, vA, x, T, L, ref_y_gpu):
                       "(OpenMP restatement of the reference's cpu_sparse)"}, err
 
 
+def calibrate_copy(mib=512, reps=5):
+    """A plain device copy of known size, profiled with the same HBM counters
+    (FETCH_SIZE, WRITE_SIZE) as the kronmult to scale them to bytes."""
+    import torch
+
+    n = mib * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
               "supported (the row-sharded multi-GPU design is described in DESIGN.md)",
@@ -95,6 +114,8 @@ def main():
     flops = mat.flops()
     nnz = mat.num_pairs()
     nconn = len(asgard_amd.connect_1d(L)[1])
+    # setup, untimed: the autotuner picks the workgroup shape and LDS layout
+    waves, lds_tables = mat.autotune(xd, yd)
 
     for _ in range(args.warmup):
         mat.apply(1.0, xd, 0.0, yd)
@@ -137,7 +158,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "kronmult apply, d=3 n=3 |l|_1<=6 sparse grid, 3 terms",
                    "cells": int(cells), "connected_pairs": int(nnz),
-                   "flops_per_apply": int(flops), "parallelism": "single GPU"},
+                   "flops_per_apply": int(flops), "parallelism": "single GPU",
+                   "waves_per_row": waves, "lds_factor_tables": lds_tables},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),

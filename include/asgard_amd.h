@@ -140,9 +140,24 @@ akm_status akm_apply_euler(akm_matrix mat, double dt, double alpha, const void *
 akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_size,
                            int64_t *output_size, int64_t *num_pairs, int64_t *flops);
 
-/* Tuning knob: columns per work chunk (0 = automatic) and whether to force the
- * block-wide LDS kernel (for tests).  Takes effect at the next apply. */
-akm_status akm_set_tuning(akm_matrix mat, int64_t chunk_columns, int force_generic);
+/* Tuning knobs: wavefronts per row workgroup (0 = automatic, else 1, 2, 4 or
+ * 8), the LDS bytes a workgroup may hold for the row's factor tables
+ * (-1 = automatic, 0 = read factors from global memory) and whether to
+ * force the block-wide LDS kernel (for tests).  Takes effect at the next
+ * apply.  No counterpart in the reference (its launch shape is fixed at
+ * build time, CMakeLists.txt:231-232). */
+akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
+                          int force_generic);
+
+/* Autotuner: times every workgroup shape (1, 2, 4, 8 wavefronts per row)
+ * with and without the LDS factor tables on the matrix itself (device x/y,
+ * y is overwritten) and keeps the fastest.  Choices are cached per
+ * (precision, d, n, terms, max level, rows, pairs) for the process, so a
+ * second matrix of the same shape reuses them without timing.  Writes the
+ * chosen waves per row and LDS switch (1 = tables) when the pointers are not
+ * NULL.  No counterpart in the reference (fixed launch shapes). */
+akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_row,
+                        int *lds_tables);
 
 /* Times `iterations` back-to-back applies (device x/y) with HIP events on the
  * matrix's stream; returns the average milliseconds per apply. */

@@ -118,11 +118,11 @@ def test_empty_rows_and_no_terms():
     assert np.allclose(host(y), -3.0 * c["y"], rtol=0, atol=0)
 
 
-@pytest.mark.parametrize("chunk", [1, 64, 200, 4096])
-def test_chunking_and_determinism(chunk):
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_workgroup_shapes_and_determinism(waves):
     c = sparse_grid_case(3, 2, 5, 2, seed=5)
     m = asgard_amd.KronmultMatrix.sparse_grid(3, 2, c["table"], 5, 2, c["vA"])
-    m.set_tuning(chunk_columns=chunk)
+    m.set_tuning(waves_per_row=waves)
     x = dev(c["x"])
     y1 = dev(c["y"])
     m.apply(1.0, x, 1.0, y1)
@@ -131,6 +131,40 @@ def test_chunking_and_determinism(chunk):
     a, b = host(y1), host(y2)
     assert np.array_equal(a, b), "apply must be bitwise repeatable"
     assert rel(a, oracle_apply(c, 1.0, 1.0)) < TOL64
+
+
+@pytest.mark.parametrize("d,n,L,T", [(3, 3, 5, 3), (4, 2, 5, 4), (6, 2, 3, 2)])
+def test_autotune_keeps_parity(d, n, L, T):
+    c = sparse_grid_case(d, n, L, T, seed=11)
+    m = asgard_amd.KronmultMatrix.sparse_grid(d, n, c["table"], L, T, c["vA"])
+    x = dev(c["x"])
+    scratch = dev(np.zeros_like(c["x"]))
+    waves, lds = m.autotune(x, scratch)
+    assert waves in (1, 2, 4, 8)
+    # a second operator of the same shape reuses the cached choice
+    m2 = asgard_amd.KronmultMatrix.sparse_grid(d, n, c["table"], L, T, c["vA"])
+    assert m2.autotune(x, scratch) == (waves, lds)
+    y = dev(c["y"])
+    m.apply(1.0, x, 1.0, y)
+    assert rel(host(y), oracle_apply(c, 1.0, 1.0)) < TOL64
+
+
+@pytest.mark.parametrize("lds", [-1, 0])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_lds_tables_and_shapes(waves, lds):
+    c = sparse_grid_case(4, 2, 5, 3, seed=12)
+    m = asgard_amd.KronmultMatrix.sparse_grid(4, 2, c["table"], 5, 3, c["vA"])
+    m.set_tuning(waves_per_row=waves, lds_cap_bytes=lds)
+    y = dev(c["y"])
+    m.apply(0.25, dev(c["x"]), -1.0, y)
+    assert rel(host(y), oracle_apply(c, 0.25, -1.0)) < TOL64
+
+
+def test_invalid_workgroup_shape():
+    c = sparse_grid_case(2, 2, 3, 1, seed=5)
+    m = asgard_amd.KronmultMatrix.sparse_grid(2, 2, c["table"], 3, 1, c["vA"])
+    with pytest.raises(ValueError):
+        m.set_tuning(waves_per_row=3)
 
 
 def test_sparse_pattern_bit_exact_small():
