@@ -29,13 +29,13 @@ HBM_PEAK_GBS = 8000.0
 
 
 def make_workload(d, n, L, T, seed=0):
-    import oracle  # CPU input generator for the grid and the 1D operators
+    from asgard_amd import grid
 
     rng = np.random.default_rng(seed)
-    table, _ = oracle.element_table([L] * d)
+    table, _ = grid.element_table([L] * d)
     size1d = n * (1 << L)
     ops = [np.asfortranarray(rng.uniform(-1, 1, (size1d, size1d))) for _ in range(T * d)]
-    vA = oracle.build_vA(d, n, T, L, ops)
+    vA = grid.build_vA(d, n, T, L, ops)
     x = rng.standard_normal(table.shape[0] * n ** d)
     return table, vA, x
 

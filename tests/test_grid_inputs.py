@@ -1,0 +1,50 @@
+"""The product's host-side input builders (asgard_amd.grid) against the
+reference fixtures and the oracle, bit for bit.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle
+from asgard_amd import grid
+from tests.fixtures import golden, read_octave
+
+
+@pytest.mark.parametrize("levels,tag", [((7,), "1d"), ((5, 2), "2d"), ((3, 2, 3), "3d")])
+@pytest.mark.parametrize("g", ["SG", "FG"])
+def test_element_table_fixture(levels, tag, g):
+    table, ids = grid.element_table(levels, max_level=7, full_grid=(g == "FG"))
+    gold_table = read_octave(golden("element_table", f"table_{tag}_{g}.dat")).astype(np.int32)
+    gold_ids = read_octave(golden("element_table", f"ids_{tag}_{g}.dat")).ravel().astype(np.int64)
+    assert np.array_equal(table, gold_table)
+    assert np.array_equal(ids, gold_ids)
+
+
+@pytest.mark.parametrize("levels", [(5, 5), (6, 6, 6), (7, 7, 7, 7), (4, 4, 4, 4, 4), (7,) * 6,
+                                    (3, 5, 2), (2, 2, 2, 2, 2, 2)])
+def test_element_table_matches_oracle(levels):
+    t, i = grid.element_table(levels)
+    to, io = oracle.element_table(levels)
+    assert np.array_equal(t, to) and np.array_equal(i, io)
+
+
+@pytest.mark.parametrize("d,n,T,L", [(2, 2, 2, 5), (3, 3, 3, 4), (1, 4, 1, 3), (2, 1, 2, 3)])
+def test_build_vA_matches_oracle(d, n, T, L):
+    rng = np.random.default_rng(0)
+    s = n * (1 << L)
+    ops = [np.asfortranarray(rng.uniform(-1, 1, (s, s))) for _ in range(T * d)]
+    assert np.array_equal(grid.build_vA(d, n, T, L, ops), oracle.build_vA(d, n, T, L, ops))
+
+
+@pytest.mark.parametrize("d,n,T,L", [(2, 2, 2, 3), (3, 3, 1, 2)])
+def test_dense_terms_match_oracle(d, n, T, L):
+    rng = np.random.default_rng(1)
+    s = n * (1 << L)
+    ops = [np.asfortranarray(rng.uniform(-1, 1, (s, s))) for _ in range(T * d)]
+    mine = grid.build_dense_terms(d, n, T, 1 << L, ops)
+    ref = oracle.build_dense_terms(d, n, T, 1 << L, ops)
+    for a, b in zip(mine, ref):
+        assert np.array_equal(a, b)
+
+
+def test_cell_1d_index_matches_oracle():
+    t, _ = grid.element_table((4, 3, 5))
+    assert np.array_equal(grid.cell_1d_index(t), oracle.cell_1d_index(t))
