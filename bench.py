@@ -67,6 +67,25 @@ def cpu_baseline(d, n, table, vA, x, T, L, ref_y_gpu):
                       "(OpenMP restatement of the reference's cpu_sparse)"}, err
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kron_f64_d3n3_L6_T3_hbm_traffic.txt")
+
+
+def profiled_traffic(waves, lds_tables):
+    """HBM bytes per apply of the chosen kernel shape, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (each counter
+    scaled by what it reads for a plain device copy of known size); None when
+    the shape was not profiled."""
+    key = "traffic_bytes W=%d lds=%d:" % (waves, 1 if lds_tables else 0)
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(key):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def calibrate_copy(mib=512, reps=5):
     """A plain device copy of known size, profiled with the same HBM counters
     (FETCH_SIZE, WRITE_SIZE) as the kronmult to scale them to bytes."""
@@ -163,7 +182,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic(waves, lds_tables),
                      "algorithmic_bytes": int(algorithmic_bytes(d, n, T, cells, nnz, nconn))},
         "cpu_baseline": cpu,
         "parity_max_rel_err_vs_oracle": parity,
