@@ -47,9 +47,25 @@ EXPORTS = (
     "akm_create_sparse_grid", "akm_create_dense", "akm_update_coefficients",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
     "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
+    "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
+    "akm_mlp_get_params", "akm_mlp_set_params", "akm_mlp_forward", "akm_mlp_gradients",
+    "akm_mlp_adam_step", "akm_mlp_train", "akm_precond_apply", "akm_precond_destroy",
 )
 
 GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
+
+
+class SolveStats(ctypes.Structure):
+    """akm_solve_stats: outcome of :func:`gmres_euler_mixed`."""
+    _fields_ = [("rel_residual", ctypes.c_double), ("abs_residual", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double), ("iterations", ctypes.c_int),
+                ("cycles", ctypes.c_int), ("applies_f64", ctypes.c_int),
+                ("applies_f32", ctypes.c_int), ("precond_applies", ctypes.c_int),
+                ("fallback", ctypes.c_int), ("fallback_iterations", ctypes.c_int),
+                ("converged", ctypes.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class AkmError(RuntimeError):
@@ -103,6 +119,20 @@ def load_library(path: str = LIB_PATH):
     L.akm_destroy.argtypes = [_vp]
     L.akm_gmres_euler.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
                                   ctypes.POINTER(_i)]
+    L.akm_gmres_euler_mixed.argtypes = [_vp, _vp, _vp, _d, _vp, _vp, _i, _i, _d, _d,
+                                        ctypes.POINTER(SolveStats)]
+    L.akm_mlp_create.argtypes = [_i, _i, _i64, _vp, _i, _i, ctypes.c_uint64, ctypes.POINTER(_vp)]
+    L.akm_mlp_info.argtypes = [_vp, ctypes.POINTER(_i64)] + [ctypes.POINTER(_i)] * 5
+    L.akm_mlp_set_stream.argtypes = [_vp, _vp]
+    L.akm_mlp_get_params.argtypes = [_vp, _vp]
+    L.akm_mlp_set_params.argtypes = [_vp, _vp]
+    L.akm_mlp_forward.argtypes = [_vp, _vp, _i64, _vp]
+    L.akm_mlp_gradients.argtypes = [_vp, _vp, _vp, _i64, ctypes.POINTER(_d), _vp]
+    L.akm_mlp_adam_step.argtypes = [_vp, _vp, _d]
+    L.akm_mlp_train.argtypes = [_vp, _vp, _vp, _i64, _i, _i, _d, ctypes.c_uint64,
+                                ctypes.POINTER(_d)]
+    L.akm_precond_apply.argtypes = [_vp, _vp, _vp, _vp]
+    L.akm_precond_destroy.argtypes = [_vp]
     for name in EXPORTS:
         if name not in ("akm_last_error", "akm_version"):
             getattr(L, name).restype = _i
@@ -361,3 +391,122 @@ def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, t
                                           int(max_iter), float(tol), dx, ctypes.byref(err),
                                           ctypes.byref(its)), allow_not_converged=True)
     return GmresInfo(err.value, its.value)
+
+
+def gmres_euler_mixed(dt, mat: KronmultMatrix, x, b, mat32: KronmultMatrix = None,
+                      precond: "MlpPreconditioner" = None, restart=-1, max_cycles=-1,
+                      tol=1e-10, inner_tol=-1.0) -> SolveStats:
+    """(I - dt*A) x = b by FP64 iterative refinement around flexible GMRES
+    cycles: FP64 residuals, basis, Gram-Schmidt and least squares; the FP32
+    operator `mat32` inside the cycles (None = `mat`); the right
+    preconditioner `precond` on every basis vector (None = none); plain FP64
+    GMRES takes over when a cycle fails to reduce the FP64 residual.  `tol` is
+    relative to |b|.  x (initial guess, overwritten) and b are float64 CUDA
+    tensors.  Returns the :class:`SolveStats`."""
+    if mat.prec != AKM_F64:
+        raise AkmInvalid(AKM_ERR_INVALID, "the operator must be FP64")
+    px, dx = _vec_ptr(x, np.float64)
+    pb, db = _vec_ptr(b, np.float64)
+    if not (dx and db):
+        raise AkmInvalid(AKM_ERR_INVALID, "gmres_euler_mixed needs device tensors")
+    st = SolveStats()
+    _check(load_library().akm_gmres_euler_mixed(
+        mat._h, mat32._h if mat32 is not None else None,
+        precond._h if precond is not None else None, float(dt), px, pb, int(restart),
+        int(max_cycles), float(tol), float(inner_tol), ctypes.byref(st)),
+        allow_not_converged=True)
+    return st
+
+
+# --------------------------------------------------- learned preconditioner ---
+
+class MlpPreconditioner:
+    """Per-cell MLP right preconditioner (akm_mlp_*): z_c = r_c + |r_c| o(x_c),
+    x_c = [r_c/|r_c|, cell features]; trained in FP32 on the GPU, applied with
+    BF16 weights on the matrix cores."""
+
+    def __init__(self, d, n, table, max_level, hidden=256, seed=0):
+        L = load_library()
+        table = _np(table, np.int32)
+        h = _vp()
+        _check(L.akm_mlp_create(int(d), int(n), table.shape[0], table.ctypes.data,
+                                int(max_level), int(hidden), int(seed), ctypes.byref(h)))
+        self._h = h
+        self.d, self.n, self.cells = int(d), int(n), table.shape[0]
+        np_ = _i64()
+        vals = [_i() for _ in range(5)]
+        _check(L.akm_mlp_info(h, ctypes.byref(np_), *[ctypes.byref(v) for v in vals]))
+        self.num_params = np_.value
+        (self.inputs, self.padded_inputs, self.hidden, self.outputs,
+         self.padded_outputs) = [v.value for v in vals]
+
+    def close(self):
+        if self._h:
+            load_library().akm_precond_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self):
+        """Offsets of W1, b1, W2, b2, W3, b3 in the flat parameter array."""
+        H, K1, N3 = self.hidden, self.padded_inputs, self.padded_outputs
+        offs = {}
+        o = 0
+        for name, size in [("W1", H * K1), ("b1", H), ("W2", H * H), ("b2", H),
+                           ("W3", N3 * H), ("b3", N3)]:
+            offs[name] = (o, size)
+            o += size
+        return offs
+
+    def set_stream(self, stream_ptr):
+        _check(load_library().akm_mlp_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    def get_params(self):
+        out = np.zeros(self.num_params, np.float32)
+        _check(load_library().akm_mlp_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, params):
+        arr = _np(params, np.float32)
+        if arr.size != self.num_params:
+            raise AkmInvalid(AKM_ERR_INVALID, "wrong parameter count")
+        _check(load_library().akm_mlp_set_params(self._h, arr.ctypes.data))
+
+    def forward(self, X, O):
+        """O (rows x N3 float32 CUDA tensor) = o(X) for X (rows x K1)."""
+        _check(load_library().akm_mlp_forward(self._h, X.data_ptr(), X.shape[0], O.data_ptr()))
+        return O
+
+    def gradients(self, X, T, grad=None):
+        """(loss, grad) of the mean squared error of o(X) against T."""
+        loss = _d()
+        _check(load_library().akm_mlp_gradients(self._h, X.data_ptr(), T.data_ptr(), X.shape[0],
+                                                ctypes.byref(loss),
+                                                grad.data_ptr() if grad is not None else None))
+        return loss.value, grad
+
+    def adam_step(self, grad, lr):
+        _check(load_library().akm_mlp_adam_step(self._h, grad.data_ptr(), float(lr)))
+
+    def train(self, residuals, corrections, epochs=10, batch=1024, lr=1e-3, seed=0):
+        """Trains on (residual, correction) pairs: float64 CUDA tensors of shape
+        (pairs, cells*n^d).  Returns the mean loss of the last epoch."""
+        R = residuals.contiguous()
+        E = corrections.contiguous()
+        if R.shape != E.shape or R.shape[-1] != self.cells * self.outputs:
+            raise AkmInvalid(AKM_ERR_INVALID, "pairs must be (count, cells*n^d) tensors")
+        loss = _d()
+        _check(load_library().akm_mlp_train(self._h, R.data_ptr(), E.data_ptr(), R.shape[0],
+                                            int(epochs), int(batch), float(lr), int(seed),
+                                            ctypes.byref(loss)))
+        return loss.value
+
+    def apply(self, v, z, stream_ptr=0):
+        """z = P(v) on float64 CUDA tensors (BF16 matrix-core inference)."""
+        _check(load_library().akm_precond_apply(self._h, v.data_ptr(), z.data_ptr(),
+                                                ctypes.c_void_p(stream_ptr)))
+        return z

@@ -1,5 +1,7 @@
 // Device-resident restarted GMRES for the implicit (backward-Euler) step
-// (I - dt*A) x = b: the algorithm of solver::simple_gmres
+// (I - dt*A) x = b.
+//
+// akm_gmres_euler is the algorithm of solver::simple_gmres
 // (src/solver.cpp:156-298) with every vector and Hessenberg operation on the
 // GPU.  Per inner iteration: one fused apply (kronmult + identity), the
 // classical Gram-Schmidt dot products V^T w, the update w -= V h fused with
@@ -8,10 +10,21 @@
 // rot/rotg semantics) and writes the residual estimate, and the scaling of w.
 // Reductions use fixed block partitions and orders, so results repeat
 // bit for bit.  The host reads one scalar per iteration to test convergence.
+//
+// akm_gmres_euler_mixed is an FP64 iterative refinement around flexible
+// GMRES cycles: the residual b - (I - dt*A)x, the Krylov basis, the
+// Gram-Schmidt and the Hessenberg least squares are FP64; the operator inside
+// a cycle may be the FP32 kronmult (vectors rounded to FP32 for the apply
+// only) and a right preconditioner may act on every basis vector.  After each
+// cycle the true FP64 residual decides: it must at least halve, else the
+// cycle is undone and plain FP64 GMRES finishes the solve.
 #include "matrix.hpp"
+#include "precond.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <limits>
 #include <vector>
 
 namespace akm
@@ -120,6 +133,25 @@ gm_update(int64_t n, int64_t rows_per_block, const P *V, int64_t ldv, P *w, int 
     sq_partial[blockIdx.x] = sq;
 }
 
+//! sq_partial[bx] = sum of x^2 over the block's rows (x is only read).
+template<typename P>
+__global__ void __launch_bounds__(gm_threads)
+gm_sumsq(int64_t n, int64_t rows_per_block, const P *x, P *sq_partial)
+{
+  __shared__ P sh[gm_threads];
+  const int64_t r0 = blockIdx.x * rows_per_block;
+  const int64_t r1 = min(n, r0 + rows_per_block);
+  P sq = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += gm_threads)
+  {
+    const P v = x[i];
+    sq = fma(v, v, sq);
+  }
+  sq = block_sum(sq, sh);
+  if (threadIdx.x == 0)
+    sq_partial[blockIdx.x] = sq;
+}
+
 template<typename P>
 __device__ __forceinline__ void dev_rot(P &x, P &y, P c, P s)
 {
@@ -153,11 +185,13 @@ __device__ __forceinline__ void dev_rotg(P &a, P b, P &c, P &s)
 //! Finishes the norm from the block partials.  With j < 0 this starts a
 //! restart cycle (g[0] = |r|); otherwise it processes Hessenberg column j:
 //! stores h, applies the previous rotations, computes the new one, the
-//! residual estimate and (when still above tol) rotates g.
+//! residual estimate and rotates g.  With rotate_always == 0, g is rotated
+//! only while the estimate is above tol, as simple_gmres does
+//! (src/solver.cpp:261-267).
 template<typename P>
 __global__ void __launch_bounds__(64)
 gm_givens(int nblk, const P *sq_partial, const P *h, int j, P *H, P *cs, P *sn, P *g, P tol,
-          gm_state<P> *st)
+          int rotate_always, gm_state<P> *st)
 {
   __shared__ P sh[64];
   P s = 0;
@@ -186,7 +220,7 @@ gm_givens(int nblk, const P *sq_partial, const P *h, int j, P *H, P *cs, P *sn, 
   sn[j] = sv;
   const P res = fabs(sv * g[j]);
   st->inner_res = res;
-  if (res > tol)
+  if (rotate_always != 0 || res > tol)
   {
     g[j + 1] = 0;
     dev_rot(g[j], g[j + 1], c, sv);
@@ -239,9 +273,26 @@ gm_xupdate(int64_t n, const P *V, int64_t ldv, int m, const P *y, P *x)
   }
 }
 
+//! y = fp32(x)
+__global__ void __launch_bounds__(gm_threads) gm_to_f32(int64_t n, const double *x, float *y)
+{
+  for (int64_t i = blockIdx.x * int64_t(gm_threads) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * gm_threads)
+    y[i] = static_cast<float>(x[i]);
+}
+
+//! w = v + c * y, y in FP32 (the identity part of I - dt*A stays FP64).
+__global__ void __launch_bounds__(gm_threads)
+gm_axpy_f32(int64_t n, const double *v, double c, const float *y, double *w)
+{
+  for (int64_t i = blockIdx.x * int64_t(gm_threads) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * gm_threads)
+    w[i] = fma(c, static_cast<double>(y[i]), v[i]);
+}
+
 } // namespace akm
 
-//! Device workspace of the solver, kept with the matrix between solves.
+//! Device workspace of the solvers, kept with the matrix between solves.
 struct akm_gmres_workspace
 {
   int restart = 0;
@@ -249,6 +300,9 @@ struct akm_gmres_workspace
   size_t esize = 8;
   akm::dev_buffer<unsigned char> V, work, xb;
   akm::dev_buffer<unsigned char> state;
+  // flexible / mixed-precision solver only
+  akm::dev_buffer<double> Z, xprev;
+  akm::dev_buffer<float> v32, y32;
   void *host_state = nullptr; // pinned
   ~akm_gmres_workspace()
   {
@@ -261,38 +315,129 @@ static void delete_workspace(akm_gmres_workspace *w) { delete w; }
 
 namespace akm
 {
+//! Rows of a vector handled by one reduction block.
+static int64_t rows_per_block_for(int64_t n) { return std::max<int64_t>(4096, (n + 1023) / 1024); }
+
+//! One restart cycle's device state and the kernel sequence of its steps.
+template<typename P>
+struct gm_cycle
+{
+  int64_t n = 0;
+  hipStream_t s = nullptr;
+  int restart = 0;
+  P *V = nullptr;
+  P *Z = nullptr; // preconditioned directions (flexible GMRES), else null
+  int64_t ldv = 0;
+  int64_t rpb = 0;
+  int nblk = 0, sblocks = 0;
+  P *h = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr;
+  P *dotp = nullptr, *sqp = nullptr;
+  gm_state<P> *st = nullptr, *hst = nullptr;
+  int rotate_always = 0;
+
+  gm_cycle(akm_gmres_workspace &ws, int64_t n_, hipStream_t s_, int restart_, P *Z_,
+           int rotate_always_)
+      : n(n_), s(s_), restart(restart_), Z(Z_), rotate_always(rotate_always_)
+  {
+    V = reinterpret_cast<P *>(ws.V.data());
+    ldv = n;
+    rpb = rows_per_block_for(n);
+    nblk = static_cast<int>((n + rpb - 1) / rpb);
+    sblocks = static_cast<int>(std::min<int64_t>((n + gm_threads - 1) / gm_threads, 2048));
+    // small work arrays: h, H (packed), cs, sn, g, dot partials, sq partials
+    h = reinterpret_cast<P *>(ws.work.data());
+    H = h + (restart + 1);
+    cs = H + int64_t(restart) * (restart + 1) / 2 + 1;
+    sn = cs + (restart + 1);
+    g = sn + (restart + 1);
+    dotp = g + (restart + 2);
+    sqp = dotp + int64_t(restart + 1) * nblk;
+    st = reinterpret_cast<gm_state<P> *>(ws.state.data());
+    hst = static_cast<gm_state<P> *>(ws.host_state);
+  }
+
+  int sync_state()
+  {
+    AKM_HIP(hipMemcpyAsync(hst, st, sizeof(gm_state<P>), hipMemcpyDeviceToHost, s));
+    AKM_HIP(hipStreamSynchronize(s));
+    return AKM_SUCCESS;
+  }
+
+  //! |x| of a vector that is only read.
+  int norm(const P *x, P &out)
+  {
+    gm_sumsq<P><<<nblk, gm_threads, 0, s>>>(n, rpb, x, sqp);
+    gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, -1, H, cs, sn, g, P{0}, rotate_always, st);
+    AKM_TRY(check_launch("gmres norm"));
+    AKM_TRY(sync_state());
+    out = hst->nrm;
+    return AKM_SUCCESS;
+  }
+
+  //! V_0 holds a residual r: g_0 = |r|, V_0 = r / |r|; beta = |r|.
+  int start(P tol, P &beta)
+  {
+    gm_sumsq<P><<<nblk, gm_threads, 0, s>>>(n, rpb, V, sqp);
+    gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, -1, H, cs, sn, g, tol, rotate_always, st);
+    gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, V, st);
+    AKM_TRY(check_launch("gmres restart"));
+    AKM_TRY(sync_state());
+    beta = hst->inner_res;
+    return AKM_SUCCESS;
+  }
+
+  //! Arnoldi step j: z_j = pre(v_j) (flexible) or v_j, w = op(z_j), CGS
+  //! against V_0..V_j, Givens; est = new residual estimate.
+  int step(int j, std::function<int(const P *, P *)> const &op,
+           std::function<int(const P *, P *)> const *pre, P tol, P &est)
+  {
+    const P *vj = V + int64_t(j) * ldv;
+    P *w = V + int64_t(j + 1) * ldv;
+    const P *zj = vj;
+    if (pre != nullptr)
+    {
+      P *zz = Z + int64_t(j) * ldv;
+      AKM_TRY((*pre)(vj, zz));
+      zj = zz;
+    }
+    AKM_TRY(op(zj, w));
+    const int ncols = j + 1;
+    dim3 grid(nblk, (ncols + gm_cols - 1) / gm_cols);
+    gm_dots<P><<<grid, gm_threads, 0, s>>>(n, rpb, V, ldv, w, ncols, dotp);
+    gm_reduce<P><<<ncols, gm_threads, 0, s>>>(nblk, dotp, h);
+    gm_update<P><<<nblk, gm_threads, sizeof(P) * ncols, s>>>(n, rpb, V, ldv, w, ncols, h, sqp);
+    gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, j, H, cs, sn, g, tol, rotate_always, st);
+    gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, w, st);
+    AKM_TRY(check_launch("gmres inner iteration"));
+    AKM_TRY(sync_state());
+    est = hst->inner_res;
+    return AKM_SUCCESS;
+  }
+
+  //! x += B[:, 0:m] H^-1 g with B = Z (flexible) or V.
+  int finish(int m, P *x, bool flexible)
+  {
+    if (m <= 0)
+      return AKM_SUCCESS;
+    gm_backsolve<P><<<1, 1, 0, s>>>(m, H, g);
+    gm_xupdate<P><<<sblocks, gm_threads, sizeof(P) * m, s>>>(n, flexible ? Z : V, ldv, m, g, x);
+    return check_launch("gmres update");
+  }
+};
+
 template<typename P>
 static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int max_outer,
                        P tol, double *err, int *iterations)
 {
   const int64_t n = m.output_size();
   hipStream_t s = m.stream;
-  auto &ws = *m.gmres;
-  P *V = reinterpret_cast<P *>(ws.V.data());
-  const int64_t ldv = n;
-  // small work arrays: h, H, cs, sn, g, dot partials, sq partials
-  const int64_t rows_per_block = std::max<int64_t>(4096, (n + 1023) / 1024);
-  const int nblk = static_cast<int>((n + rows_per_block - 1) / rows_per_block);
-  const int ncolblk_max = (restart + 1 + gm_cols - 1) / gm_cols;
-  P *h = reinterpret_cast<P *>(ws.work.data());
-  P *H = h + (restart + 1);
-  P *cs = H + int64_t(restart) * (restart + 1) / 2 + 1;
-  P *sn = cs + (restart + 1);
-  P *g = sn + (restart + 1);
-  P *dotp = g + (restart + 2);
-  P *sqp = dotp + int64_t(restart + 1) * nblk;
-  (void)ncolblk_max;
-  auto *st = reinterpret_cast<gm_state<P> *>(ws.state.data());
-  auto *hst = static_cast<gm_state<P> *>(ws.host_state);
-  const int sblocks = static_cast<int>(std::min<int64_t>((n + gm_threads - 1) / gm_threads, 2048));
-
-  auto read_state = [&]() -> int {
-    AKM_HIP(hipMemcpyAsync(hst, st, sizeof(gm_state<P>), hipMemcpyDeviceToHost, s));
-    AKM_HIP(hipStreamSynchronize(s));
-    return AKM_SUCCESS;
+  gm_cycle<P> cyc(*m.gmres, n, s, restart, nullptr, 0);
+  P *V = cyc.V;
+  const std::function<int(const P *, P *)> op = [&](const P *v, P *w) -> int {
+    return apply_device(m, -dt, v, P{0}, w, P{1}, v);
   };
 
-  int total = 0, outer = 0, inner = 0;
+  int total = 0, outer = 0;
   P inner_res = 0, outer_res = tol + 1;
   while (outer_res > tol && outer < max_outer)
   {
@@ -300,39 +445,15 @@ static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int
     AKM_HIP(hipMemcpyAsync(V, b, sizeof(P) * n, hipMemcpyDeviceToDevice, s));
     AKM_TRY(apply_device(m, -dt * P{-1}, x, P{1}, V, P{-1}, x));
     ++total;
-    gm_update<P><<<nblk, gm_threads, 0, s>>>(n, rows_per_block, V, ldv, V, 0, h, sqp);
-    gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, -1, H, cs, sn, g, tol, st);
-    gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, V, st);
-    AKM_TRY(check_launch("gmres restart"));
-    AKM_TRY(read_state());
-    inner_res = hst->inner_res;
-
-    inner = 0;
+    AKM_TRY(cyc.start(tol, inner_res));
+    int inner = 0;
     while (inner_res > tol && inner < restart)
     {
-      const P *vj = V + int64_t(inner) * ldv;
-      P *w = V + int64_t(inner + 1) * ldv;
-      AKM_TRY(apply_device(m, -dt, vj, P{0}, w, P{1}, vj));
+      AKM_TRY(cyc.step(inner, op, nullptr, tol, inner_res));
       ++total;
-      const int ncols = inner + 1;
-      dim3 grid(nblk, (ncols + gm_cols - 1) / gm_cols);
-      gm_dots<P><<<grid, gm_threads, 0, s>>>(n, rows_per_block, V, ldv, w, ncols, dotp);
-      gm_reduce<P><<<ncols, gm_threads, 0, s>>>(nblk, dotp, h);
-      gm_update<P><<<nblk, gm_threads, sizeof(P) * ncols, s>>>(n, rows_per_block, V, ldv, w,
-                                                               ncols, h, sqp);
-      gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, inner, H, cs, sn, g, tol, st);
-      gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, w, st);
-      AKM_TRY(check_launch("gmres inner iteration"));
-      AKM_TRY(read_state());
-      inner_res = hst->inner_res;
       ++inner;
     }
-    if (inner > 0)
-    {
-      gm_backsolve<P><<<1, 1, 0, s>>>(inner, H, g);
-      gm_xupdate<P><<<sblocks, gm_threads, sizeof(P) * inner, s>>>(n, V, ldv, inner, g, x);
-      AKM_TRY(check_launch("gmres update"));
-    }
+    AKM_TRY(cyc.finish(inner, x, false));
     ++outer;
     outer_res = inner_res;
   }
@@ -342,6 +463,103 @@ static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int
   if (iterations)
     *iterations = total;
   return (outer_res > tol) ? AKM_ERR_NOT_CONVERGED : AKM_SUCCESS;
+}
+
+//! The mixed-precision / flexible solver (FP64 matrix m64).
+static int gmres_mixed(akm_matrix_s &m64, akm_matrix_s *m32, akm_precond_s *pre, double dt,
+                       double *x, const double *b, int restart, int max_cycles, double tol_rel,
+                       double inner_rel, akm_solve_stats &stats)
+{
+  const int64_t n = m64.output_size();
+  hipStream_t s = m64.stream;
+  auto &ws = *m64.gmres;
+  double *Z = (pre != nullptr) ? ws.Z.data() : nullptr;
+  gm_cycle<double> cyc(ws, n, s, restart, Z, 1);
+  double *V = cyc.V;
+  const int sblocks = cyc.sblocks;
+
+  std::function<int(const double *, double *)> op64 = [&](const double *v, double *w) -> int {
+    ++stats.applies_f64;
+    return apply_device(m64, -dt, v, 0.0, w, 1.0, v);
+  };
+  std::function<int(const double *, double *)> op32 = [&](const double *v, double *w) -> int {
+    ++stats.applies_f32;
+    gm_to_f32<<<sblocks, gm_threads, 0, s>>>(n, v, ws.v32.data());
+    AKM_TRY(check_launch("gmres fp32 cast"));
+    AKM_TRY(apply_device(*m32, 1.0, ws.v32.data(), 0.0, ws.y32.data(), 0.0, nullptr));
+    gm_axpy_f32<<<sblocks, gm_threads, 0, s>>>(n, v, -dt, ws.y32.data(), w);
+    return check_launch("gmres fp32 axpy");
+  };
+  std::function<int(const double *, double *)> precond = [&](const double *v,
+                                                             double *z) -> int {
+    ++stats.precond_applies;
+    return pre->apply(v, z, s);
+  };
+  // r = b - (I - dt A) x in FP64 into V_0, normalised; beta = |r|
+  auto true_residual = [&](double &beta) -> int {
+    AKM_HIP(hipMemcpyAsync(V, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    ++stats.applies_f64;
+    AKM_TRY(apply_device(m64, dt, x, 1.0, V, -1.0, x));
+    return cyc.start(0.0, beta);
+  };
+
+  double bnorm = 0;
+  AKM_TRY(cyc.norm(b, bnorm));
+  stats.rhs_norm = bnorm;
+  if (bnorm == 0.0)
+  {
+    // the solution of (I - dt A) x = 0 is x = 0
+    AKM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    AKM_HIP(hipStreamSynchronize(s));
+    stats.converged = 1;
+    return AKM_SUCCESS;
+  }
+  const double tol_abs = tol_rel * bnorm;
+  const auto &inner_op = (m32 != nullptr) ? op32 : op64;
+  // with an FP32 operator a cycle cannot resolve the FP64 residual below
+  // ~FP32 epsilon, so it stops at inner_rel of its starting residual
+  const double cycle_rel = (m32 != nullptr) ? inner_rel : 0.0;
+
+  double beta = 0;
+  AKM_TRY(true_residual(beta));
+  bool fallback = false;
+  while (beta > tol_abs && stats.cycles < max_cycles)
+  {
+    const double stop = std::max(0.5 * tol_abs, cycle_rel * beta);
+    if (!fallback)
+      AKM_HIP(hipMemcpyAsync(ws.xprev.data(), x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    const auto &op = fallback ? op64 : inner_op;
+    const auto *pc = (fallback || pre == nullptr) ? nullptr : &precond;
+    double est = beta;
+    int inner = 0;
+    while (est > stop && inner < restart)
+    {
+      AKM_TRY(cyc.step(inner, op, pc, stop, est));
+      ++inner;
+    }
+    AKM_TRY(cyc.finish(inner, x, pc != nullptr));
+    stats.iterations += inner;
+    if (fallback)
+      stats.fallback_iterations += inner;
+    ++stats.cycles;
+    double beta_new = 0;
+    AKM_TRY(true_residual(beta_new));
+    if (!fallback && !(beta_new <= 0.5 * beta))
+    {
+      // the cycle did not halve the FP64 residual: undo it, go plain FP64
+      AKM_HIP(hipMemcpyAsync(x, ws.xprev.data(), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      fallback = true;
+      stats.fallback = 1;
+      AKM_TRY(true_residual(beta));
+      continue;
+    }
+    beta = beta_new;
+  }
+  AKM_HIP(hipStreamSynchronize(s));
+  stats.abs_residual = beta;
+  stats.rel_residual = beta / bnorm;
+  stats.converged = (beta <= tol_abs) ? 1 : 0;
+  return stats.converged ? AKM_SUCCESS : AKM_ERR_NOT_CONVERGED;
 }
 
 //! default_gmres_restarts (src/solver.cpp:142-152): clamp(512 MB / size, min(10, n), min(200, n)).
@@ -367,8 +585,8 @@ static int ensure_workspace(akm_matrix_s &m, int restart)
   ws.n = n;
   ws.esize = m.esize;
   AKM_TRY(ws.V.allocate(n * (restart + 1) * int64_t(m.esize)));
-  const int64_t rows_per_block = std::max<int64_t>(4096, (n + 1023) / 1024);
-  const int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+  const int64_t rpb = rows_per_block_for(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   const int64_t small = 4 * int64_t(restart + 2) + int64_t(restart) * (restart + 1) / 2 + 2 +
                         int64_t(restart + 1) * nblk + nblk + 16;
   AKM_TRY(ws.work.allocate(small * int64_t(m.esize)));
@@ -376,6 +594,11 @@ static int ensure_workspace(akm_matrix_s &m, int restart)
   AKM_TRY(ws.xb.allocate(2 * n * int64_t(m.esize)));
   if (ws.host_state == nullptr)
     AKM_HIP(hipHostMalloc(&ws.host_state, sizeof(gm_state<double>), hipHostMallocDefault));
+  // buffers of the flexible / mixed solver are sized on first use
+  ws.Z.reset();
+  ws.xprev.reset();
+  ws.v32.reset();
+  ws.y32.reset();
   return AKM_SUCCESS;
 }
 
@@ -431,5 +654,62 @@ extern "C" akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const 
     AKM_HIP(hipMemcpyAsync(x, xd, bytes, hipMemcpyDeviceToHost, mat->stream));
     AKM_HIP(hipStreamSynchronize(mat->stream));
   }
+  return rc;
+}
+
+extern "C" akm_status akm_gmres_euler_mixed(akm_matrix mat, akm_matrix mat32, akm_precond pre,
+                                            double dt, double *x, const double *b, int restart,
+                                            int max_cycles, double tol, double inner_tol,
+                                            akm_solve_stats *stats)
+{
+  if (mat == nullptr || x == nullptr || b == nullptr)
+    return set_error(AKM_ERR_INVALID, "gmres: null argument");
+  if (mat->prec != AKM_F64)
+    return set_error(AKM_ERR_INVALID, "gmres mixed: the operator must be FP64");
+  if (mat->rows != mat->cols)
+    return set_error(AKM_ERR_INVALID, "gmres: the operator must be square");
+  const int64_t n = mat->output_size();
+  if (n <= 0)
+    return set_error(AKM_ERR_INVALID, "gmres: empty system");
+  if (mat32 != nullptr &&
+      (mat32->prec != AKM_F32 || mat32->output_size() != n || mat32->input_size() != n))
+    return set_error(AKM_ERR_INVALID,
+                     "gmres mixed: the inner operator must be FP32 and of the same size");
+  if (pre != nullptr && pre->size() != n)
+    return set_error(AKM_ERR_INVALID, "gmres mixed: preconditioner size %lld != system size %lld",
+                     static_cast<long long>(pre->size()), static_cast<long long>(n));
+  if (tol <= 0)
+    tol = 1e-10;
+  if (inner_tol <= 0)
+    inner_tol = 1e-6;
+  if (restart <= 0)
+    restart = default_restarts(n, mat->esize);
+  if (restart > n)
+    return set_error(AKM_ERR_INVALID, "Number of inner iterations %d must be less than %lld!",
+                     restart, static_cast<long long>(n));
+  if (max_cycles <= 0)
+    max_cycles = static_cast<int>(std::min<int64_t>(n, 1 << 30));
+  AKM_TRY(ensure_workspace(*mat, restart));
+  auto &ws = *mat->gmres;
+  if (pre != nullptr && ws.Z.size() != n * restart)
+    AKM_TRY(ws.Z.allocate(n * restart));
+  AKM_TRY(ws.xprev.allocate(n));
+  if (mat32 != nullptr)
+  {
+    AKM_TRY(ws.v32.allocate(n));
+    AKM_TRY(ws.y32.allocate(n));
+  }
+  akm_solve_stats local{};
+  hipStream_t saved = nullptr;
+  if (mat32 != nullptr)
+  {
+    saved = mat32->stream;
+    mat32->stream = mat->stream; // one stream for the whole solve
+  }
+  int rc = gmres_mixed(*mat, mat32, pre, dt, x, b, restart, max_cycles, tol, inner_tol, local);
+  if (mat32 != nullptr)
+    mat32->stream = saved;
+  if (stats != nullptr)
+    *stats = local;
   return rc;
 }

@@ -86,6 +86,141 @@ def profiled_traffic(waves, lds_tables):
     return None
 
 
+SOLVE_WORKLOADS = {
+    "d4n2": dict(workload="implicit step (I - dt A) u = b, d=4 n=2 |l|_1<=7, 4 terms "
+                          "(advection in 2 dims, diffusion nu=0.1 in 2), dt=1e-3, GMRES(50), "
+                          "b = (I - dt A) u0, u0 ~ U[-1,1]",
+                 d=4, n=2, L=7, dt=1e-3, restart=50, rhs="operator", cpu=True),
+    "d6n2": dict(workload="implicit step (I - dt A) u = u0, d=6 n=2 |l|_1<=7, 3x+3v Vlasov "
+                          "advection (6 terms, central fluxes), dt=5e-4, GMRES(50), "
+                          "u0 ~ U[-1,1]",
+                 d=6, n=2, L=7, dt=5e-4, restart=50, rhs="state", cpu=False),
+}
+
+
+def solve_operators(key):
+    from asgard_amd import dg
+
+    w = SOLVE_WORKLOADS[key]
+    if key == "d4n2":
+        return dg.operators_advection_diffusion(w["d"], w["n"], w["L"], n_adv=2, nu=0.1)
+    return dg.operators_vlasov(w["n"], w["L"])
+
+
+def timed(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
+    """Time to a 1e-10 relative FP64 residual of one implicit step: plain FP64
+    GPU GMRES (the reference algorithm, absolute tolerance 1e-10*|b|) against
+    FP64 refinement around FP32-kronmult GMRES cycles, with and without the
+    learned preconditioner.  Each solve is run once untimed (workspace
+    allocation) and then timed from the same initial guess x0 = b."""
+    import torch
+
+    import asgard_amd
+    from asgard_amd import grid, learned
+
+    w = SOLVE_WORKLOADS[key]
+    d, n, L, dt, restart = w["d"], w["n"], w["L"], w["dt"], w["restart"]
+    table, _ = grid.element_table([L] * d)
+    ops = solve_operators(key)
+    T = len(ops) // d
+    vA = grid.build_vA(d, n, T, L, ops)
+    m64 = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
+    m32 = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA.astype(np.float32),
+                                                dtype=np.float32)
+    stream = torch.cuda.current_stream().cuda_stream
+    m64.set_stream(stream)
+    m32.set_stream(stream)
+    size = m64.output_size()
+    rng = np.random.default_rng(0)
+    u0 = torch.from_numpy(rng.uniform(-1.0, 1.0, size)).cuda()
+    scratch = torch.zeros_like(u0)
+    m64.autotune(u0, scratch)
+    m32.autotune(u0.float(), scratch.float())
+    if w["rhs"] == "operator":
+        b = m64.apply_euler(dt, 1.0, u0, 0.0, torch.zeros_like(u0))
+    else:
+        b = u0.clone()
+    torch.cuda.synchronize()
+    bnorm = float(torch.linalg.norm(b).item())
+
+    def true_rel(x):
+        r = m64.apply_euler(dt, 1.0, x, 0.0, torch.zeros_like(x))
+        return float(torch.linalg.norm(b - r).item()) / bnorm
+
+    out = {"workload": w["workload"], "unknowns": int(size), "pairs": int(m64.num_pairs()),
+           "flops_per_apply": int(m64.flops())}
+    # plain FP64 GPU GMRES
+    plain = {}
+    for rep in range(2):
+        x = b.clone()
+        info, t = timed(lambda: asgard_amd.simple_gmres_euler(dt, m64, x, b, restart, -1,
+                                                              1e-10 * bnorm))
+    plain.update(seconds=round(t, 4), applies_f64=info.iterations, rel_residual=true_rel(x))
+    x_plain = x
+    out["plain_fp64"] = plain
+
+    def mixed(mat32, pre, label):
+        for rep in range(2):
+            xm = b.clone()
+            st, t = timed(lambda: asgard_amd.gmres_euler_mixed(dt, m64, xm, b, mat32=mat32,
+                                                               precond=pre, restart=restart,
+                                                               tol=1e-10))
+        rec = st.as_dict()
+        rec.update(seconds=round(t, 4), speedup_vs_plain=round(plain["seconds"] / t, 3),
+                   rel_residual_check=true_rel(xm),
+                   rel_diff_vs_plain=float(torch.linalg.norm(xm - x_plain).item() /
+                                           torch.linalg.norm(x_plain).item()))
+        out[label] = rec
+        return xm
+
+    mixed(m32, None, "fp32_refinement")
+    (pre, tinfo), t_train = timed(lambda: learned.train_preconditioner(
+        m64, table, L, dt, samples=samples, epochs=epochs, batch=1024, lr=1e-3, hidden=256,
+        seed=0))
+    tinfo["seconds"] = round(t_train, 3)
+    out["training"] = tinfo
+    mixed(m32, pre, "fp32_refinement_learned")
+    mixed(None, None, "fp64_flexible")
+    mixed(None, pre, "fp64_flexible_learned")
+    if w["rhs"] == "operator":
+        out["max_abs_error_vs_u0"] = {
+            k: float((xx - u0).abs().max().item()) for k, xx in [("plain_fp64", x_plain)]}
+    if with_cpu and w["cpu"]:
+        out["cpu_baseline"] = solve_cpu_baseline(d, n, L, T, table, vA, dt, b.cpu().numpy(),
+                                                 restart, x_plain.cpu().numpy())
+    return out
+
+
+def solve_cpu_baseline(d, n, L, T, table, vA, dt, b, restart, x_gpu):
+    """The oracle's restatement of simple_gmres_euler with its OpenMP kronmult
+    on the host cores, same system, same tolerance; and the GPU solution's
+    difference from it."""
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_num_threads(threads)
+    pntr, indx, iA = oracle.sparse_pattern(table, n=n, T=T, max_level=L)
+    x = b.copy()
+    bn = float(np.linalg.norm(b))
+    t0 = time.perf_counter()
+    x, err, its = oracle.gmres_euler_sparse(d, n, pntr, indx, T, iA, vA, dt, x, b, restart,
+                                            1000, 1e-10 * bn)
+    t = time.perf_counter() - t0
+    return {"seconds": round(t, 3), "cores": threads, "kind": "port", "applies": its,
+            "sample": "the whole solve (oracle restatement of simple_gmres_euler)",
+            "rel_diff_gpu_plain_vs_oracle": float(np.linalg.norm(x_gpu - x) / np.linalg.norm(x))}
+
+
 def calibrate_copy(mib=512, reps=5):
     """A plain device copy of known size, profiled with the same HBM counters
     (FETCH_SIZE, WRITE_SIZE) as the kronmult to scale them to bytes."""
@@ -108,9 +243,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 512 MiB device copy (HBM counter calibration)")
+    ap.add_argument("--solve", default="d4n2,d6n2",
+                    help="implicit-step solve comparisons to run (comma list of "
+                         + ", ".join(SOLVE_WORKLOADS) + "; 'none' to skip)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="print only the solve comparisons (one JSON line each)")
+    ap.add_argument("--samples", type=int, default=200_000,
+                    help="per-cell training samples of the learned preconditioner")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    solves = [] if args.solve == "none" else [k for k in args.solve.split(",") if k]
+    if args.solve_only:
+        for k in solves:
+            print(json.dumps({"solve": k, **solve_comparison(k, samples=args.samples,
+                                                            with_cpu=not args.no_cpu)}),
+                  flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
@@ -187,6 +336,14 @@ def main():
         "cpu_baseline": cpu,
         "parity_max_rel_err_vs_oracle": parity,
     }
+    if solves:
+        line["solve"] = {}
+        for k in solves:
+            try:
+                line["solve"][k] = solve_comparison(k, samples=args.samples,
+                                                    with_cpu=not args.no_cpu)
+            except Exception as e:  # report, keep the kronmult line
+                line["solve"][k] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line))
 
 

@@ -181,6 +181,108 @@ akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, in
                            int max_iter, double tol, int on_device, double *err,
                            int *iterations);
 
+/* A right preconditioner of the flexible solver (see akm_mlp_create). */
+typedef struct akm_precond_s *akm_precond;
+
+/* Outcome of akm_gmres_euler_mixed. */
+typedef struct
+{
+  double rel_residual;     /* |b - (I - dt*A)x| / |b| with the FP64 operator, at exit */
+  double abs_residual;     /* |b - (I - dt*A)x| */
+  double rhs_norm;         /* |b| */
+  int iterations;          /* Krylov iterations over all cycles */
+  int cycles;              /* refinement (restart) cycles */
+  int applies_f64;         /* FP64 operator applications (residuals included) */
+  int applies_f32;         /* FP32 operator applications */
+  int precond_applies;     /* preconditioner applications */
+  int fallback;            /* 1 if plain FP64 GMRES took over */
+  int fallback_iterations; /* Krylov iterations of the plain FP64 fallback */
+  int converged;           /* 1 if rel_residual <= tol */
+} akm_solve_stats;
+
+/* The implicit step (I - dt*A) x = b of simple_gmres_euler
+ * (src/solver.cpp:119-135) by FP64 iterative refinement around flexible
+ * GMRES(restart) cycles.  mat is the FP64 operator; it computes every
+ * residual, and the Krylov basis, Gram-Schmidt and Hessenberg least squares
+ * are FP64.  Inside a cycle the operator is mat32 (an FP32 matrix of the same
+ * operator; NULL = mat itself) and, when pre is not NULL, every basis vector
+ * goes through the right preconditioner first (flexible GMRES).  A cycle with
+ * mat32 stops once its residual estimate is inner_tol times its starting
+ * residual.  After each cycle the FP64 residual must at least halve; if it
+ * does not, the cycle is undone and plain FP64 GMRES continues to the end.  The
+ * tolerance is relative: |b - (I - dt*A)x| <= tol*|b| with the FP64
+ * operator, which is also how the answer is judged.  x (initial guess in,
+ * solution out) and b are FP64 device vectors.  tol <= 0 selects 1e-10,
+ * inner_tol <= 0 1e-6, restart <= 0 default_gmres_restarts, max_cycles <= 0
+ * n.  Returns AKM_ERR_NOT_CONVERGED (1) when max_cycles ran out.
+ * Extends the reference, whose GMRES runs in one precision and without a
+ * preconditioner (the euler operator's preconditioner is a no-op,
+ * src/solver.cpp:133). */
+akm_status akm_gmres_euler_mixed(akm_matrix mat, akm_matrix mat32, akm_precond pre, double dt,
+                                 double *x, const double *b, int restart, int max_cycles,
+                                 double tol, double inner_tol, akm_solve_stats *stats);
+
+/* ------------------------------------------- learned preconditioner -- */
+/* No counterpart in the reference.  A per-cell MLP right preconditioner:
+ * for cell c with residual block r_c (n^d values) and s = |r_c| it reads
+ * [r_c/s, f_c] (f_c: per dimension (level + (position+1/2)/cells of the
+ * level)/(max_level+1)), has two ReLU hidden layers of `hidden` units and a
+ * linear output o, and returns z_c = r_c + s*o.  Parameters are one FP32
+ * array [W1 (H x K1) | b1 (H) | W2 (H x H) | b2 (H) | W3 (N3 x H) | b3 (N3)],
+ * W row-major (output x input), K1 = roundup16(n^d + d), N3 = roundup16(n^d);
+ * padding rows/columns are zero.  Training runs in FP32; the preconditioner
+ * runs with BF16 weights on the matrix cores, FP32 accumulation. */
+
+/* Creates the network for the grid `table` (num_cells x 2d host ints, as
+ * akm_create_sparse_grid takes it): He-normal hidden layers from `seed`,
+ * zero biases, zero output layer (identity preconditioner).  hidden <= 0
+ * selects 256 (multiple of 16, <= 512); n^d <= 256. */
+akm_status akm_mlp_create(int d, int n, int64_t num_cells, const int *table, int max_level,
+                          int hidden, uint64_t seed, akm_precond *pre);
+
+/* Parameter count and layer widths (inputs n^d + d, padded K1, hidden,
+ * outputs n^d, padded N3). */
+akm_status akm_mlp_info(akm_precond pre, int64_t *num_params, int *inputs, int *padded_inputs,
+                        int *hidden, int *outputs, int *padded_outputs);
+
+/* Stream of the training and parameter calls (hipStream_t, NULL = null stream). */
+akm_status akm_mlp_set_stream(akm_precond pre, void *stream);
+
+/* Copies the FP32 parameters to / from a host array of num_params floats. */
+akm_status akm_mlp_get_params(akm_precond pre, float *params);
+akm_status akm_mlp_set_params(akm_precond pre, const float *params);
+
+/* FP32 forward: O (rows x N3, device) = network outputs o for the inputs X
+ * (rows x K1, device). */
+akm_status akm_mlp_forward(akm_precond pre, const float *X, int64_t rows, float *O);
+
+/* Mean squared error over rows x n^d outputs of o(X) against targets T
+ * (rows x N3, device), and its gradient with respect to every parameter
+ * (num_params floats, device; may be NULL). */
+akm_status akm_mlp_gradients(akm_precond pre, const float *X, const float *T, int64_t rows,
+                             double *loss, float *grad);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the gradient
+ * `grad` (device, num_params floats); the step counter is kept by pre. */
+akm_status akm_mlp_adam_step(akm_precond pre, const float *grad, double lr);
+
+/* Trains on num_vectors (residual, correction) pairs of whole-grid FP64
+ * device vectors (num_vectors x num_cells*n^d each, pair q at offset
+ * q*num_cells*n^d): every cell of every pair is one sample with input
+ * [r_c/s, f_c] and target (e_c - r_c)/s.  Shuffled mini-batches of `batch`
+ * samples, Adam with learning rate lr, `epochs` passes; writes the mean loss
+ * of the last epoch.  Refreshes the BF16 weights. */
+akm_status akm_mlp_train(akm_precond pre, const double *residuals, const double *corrections,
+                         int64_t num_vectors, int epochs, int batch, double lr, uint64_t seed,
+                         double *final_loss);
+
+/* z = P(v) for FP64 device vectors of num_cells*n^d values, enqueued on
+ * `stream` (hipStream_t, NULL = null stream). */
+akm_status akm_precond_apply(akm_precond pre, const double *v, double *z, void *stream);
+
+/* Releases the preconditioner. */
+akm_status akm_precond_destroy(akm_precond pre);
+
 #ifdef __cplusplus
 }
 #endif
