@@ -246,7 +246,10 @@ __global__ void gm_backsolve(int m, const P *H, P *g)
     return;
   for (int j = m - 1; j >= 0; j--)
   {
-    g[j] /= H[j + (j * (j + 1)) / 2];
+    const P hjj = H[j + (j * (j + 1)) / 2];
+    // a zero diagonal only follows an exact breakdown (residual already
+    // zero); that column then takes no part in the solution
+    g[j] = (hjj != P{0}) ? g[j] / hjj : P{0};
     const P gj = g[j];
     for (int i = 0; i < j; i++)
       g[i] -= gj * H[i + (j * (j + 1)) / 2];
@@ -271,6 +274,15 @@ gm_xupdate(int64_t n, const P *V, int64_t ldv, int m, const P *y, P *x)
       s = fma(V[k * ldv + i], ys[k], s);
     x[i] += s;
   }
+}
+
+//! r = b - r
+template<typename P>
+__global__ void __launch_bounds__(gm_threads) gm_rsub(int64_t n, const P *b, P *r)
+{
+  for (int64_t i = blockIdx.x * int64_t(gm_threads) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * gm_threads)
+    r[i] = b[i] - r[i];
 }
 
 //! y = fp32(x)
@@ -303,11 +315,15 @@ struct akm_gmres_workspace
   // flexible / mixed-precision solver only
   akm::dev_buffer<double> Z, xprev;
   akm::dev_buffer<float> v32, y32;
-  void *host_state = nullptr; // pinned
+  void *host_state = nullptr; // pinned, two slots
+  hipEvent_t state_ready[2] = {nullptr, nullptr};
   ~akm_gmres_workspace()
   {
     if (host_state != nullptr)
       (void)hipHostFree(host_state);
+    for (auto &e : state_ready)
+      if (e != nullptr)
+        (void)hipEventDestroy(e);
   }
 };
 
@@ -333,6 +349,7 @@ struct gm_cycle
   P *h = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr;
   P *dotp = nullptr, *sqp = nullptr;
   gm_state<P> *st = nullptr, *hst = nullptr;
+  hipEvent_t *ready = nullptr;
   int rotate_always = 0;
 
   gm_cycle(akm_gmres_workspace &ws, int64_t n_, hipStream_t s_, int restart_, P *Z_,
@@ -354,6 +371,7 @@ struct gm_cycle
     sqp = dotp + int64_t(restart + 1) * nblk;
     st = reinterpret_cast<gm_state<P> *>(ws.state.data());
     hst = static_cast<gm_state<P> *>(ws.host_state);
+    ready = ws.state_ready;
   }
 
   int sync_state()
@@ -391,6 +409,14 @@ struct gm_cycle
   int step(int j, std::function<int(const P *, P *)> const &op,
            std::function<int(const P *, P *)> const *pre, P tol, P &est)
   {
+    AKM_TRY(launch_step(j, op, pre, tol));
+    return wait_step(j, est);
+  }
+
+  //! Enqueues Arnoldi step j and the copy of its state into host slot j % 2.
+  int launch_step(int j, std::function<int(const P *, P *)> const &op,
+                  std::function<int(const P *, P *)> const *pre, P tol)
+  {
     const P *vj = V + int64_t(j) * ldv;
     P *w = V + int64_t(j + 1) * ldv;
     const P *zj = vj;
@@ -409,8 +435,16 @@ struct gm_cycle
     gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, j, H, cs, sn, g, tol, rotate_always, st);
     gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, w, st);
     AKM_TRY(check_launch("gmres inner iteration"));
-    AKM_TRY(sync_state());
-    est = hst->inner_res;
+    AKM_HIP(hipMemcpyAsync(hst + (j & 1), st, sizeof(gm_state<P>), hipMemcpyDeviceToHost, s));
+    AKM_HIP(hipEventRecord(ready[j & 1], s));
+    return AKM_SUCCESS;
+  }
+
+  //! Waits for step j's state; est = its residual estimate.
+  int wait_step(int j, P &est)
+  {
+    AKM_HIP(hipEventSynchronize(ready[j & 1]));
+    est = hst[j & 1].inner_res;
     return AKM_SUCCESS;
   }
 
@@ -521,7 +555,19 @@ static int gmres_mixed(akm_matrix_s &m64, akm_matrix_s *m32, akm_precond_s *pre,
   const double cycle_rel = (m32 != nullptr) ? inner_rel : 0.0;
 
   double beta = 0;
-  AKM_TRY(true_residual(beta));
+  if (m32 != nullptr)
+  {
+    // the first cycle only needs its residual to the accuracy of its own
+    // target (inner_rel of it): b - x in FP64 and dt*A*x with the FP32 operator
+    AKM_TRY(op32(x, V));
+    gm_rsub<double><<<sblocks, gm_threads, 0, s>>>(n, b, V);
+    AKM_TRY(check_launch("gmres residual"));
+    AKM_TRY(cyc.start(0.0, beta));
+  }
+  else
+  {
+    AKM_TRY(true_residual(beta));
+  }
   bool fallback = false;
   while (beta > tol_abs && stats.cycles < max_cycles)
   {
@@ -530,6 +576,8 @@ static int gmres_mixed(akm_matrix_s &m64, akm_matrix_s *m32, akm_precond_s *pre,
       AKM_HIP(hipMemcpyAsync(ws.xprev.data(), x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     const auto &op = fallback ? op64 : inner_op;
     const auto *pc = (fallback || pre == nullptr) ? nullptr : &precond;
+    // the host reads every step's estimate before the next step: a step
+    // costs a whole operator apply, far more than the round trip
     double est = beta;
     int inner = 0;
     while (est > stop && inner < restart)
@@ -593,7 +641,10 @@ static int ensure_workspace(akm_matrix_s &m, int restart)
   AKM_TRY(ws.state.allocate(sizeof(gm_state<double>)));
   AKM_TRY(ws.xb.allocate(2 * n * int64_t(m.esize)));
   if (ws.host_state == nullptr)
-    AKM_HIP(hipHostMalloc(&ws.host_state, sizeof(gm_state<double>), hipHostMallocDefault));
+    AKM_HIP(hipHostMalloc(&ws.host_state, 2 * sizeof(gm_state<double>), hipHostMallocDefault));
+  for (auto &e : ws.state_ready)
+    if (e == nullptr)
+      AKM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // buffers of the flexible / mixed solver are sized on first use
   ws.Z.reset();
   ws.xprev.reset();
