@@ -50,6 +50,7 @@ EXPORTS = (
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
     "akm_mlp_get_params", "akm_mlp_set_params", "akm_mlp_forward", "akm_mlp_gradients",
     "akm_mlp_adam_step", "akm_mlp_train", "akm_precond_apply", "akm_precond_destroy",
+    "akm_batch_create", "akm_batch_info", "akm_batch_solve", "akm_batch_destroy",
 )
 
 GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
@@ -133,6 +134,10 @@ def load_library(path: str = LIB_PATH):
                                 ctypes.POINTER(_d)]
     L.akm_precond_apply.argtypes = [_vp, _vp, _vp, _vp]
     L.akm_precond_destroy.argtypes = [_vp]
+    L.akm_batch_create.argtypes = [_i, _vp, _vp, _i, ctypes.POINTER(_vp)]
+    L.akm_batch_info.argtypes = [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]
+    L.akm_batch_solve.argtypes = [_vp, _vp, _vp, _i, _d, _vp, _vp, _vp]
+    L.akm_batch_destroy.argtypes = [_vp]
     for name in EXPORTS:
         if name not in ("akm_last_error", "akm_version"):
             getattr(L, name).restype = _i
@@ -510,3 +515,59 @@ class MlpPreconditioner:
         _check(load_library().akm_precond_apply(self._h, v.data_ptr(), z.data_ptr(),
                                                 ctypes.c_void_p(stream_ptr)))
         return z
+
+
+# ------------------------------------------------------------ batched solves ---
+
+class BatchSolver:
+    """A batch of independent implicit steps (I - dt_s A_s) x_s = b_s solved
+    together (akm_batch_*): lockstep restarted GMRES, one persistent launch
+    per (d, n) group per operator apply, hipGraph-captured cycles."""
+
+    def __init__(self, mats, dts, restart=50):
+        L = load_library()
+        self._mats = list(mats)  # the matrices must outlive the batch
+        handles = (ctypes.c_void_p * len(self._mats))(*[m._h.value for m in self._mats])
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        if dt.size != len(self._mats):
+            raise AkmInvalid(AKM_ERR_INVALID, "one dt per system")
+        h = _vp()
+        _check(L.akm_batch_create(len(self._mats), handles, dt.ctypes.data, int(restart),
+                                  ctypes.byref(h)))
+        self._h = h
+        self.count = len(self._mats)
+
+    def close(self):
+        if self._h:
+            load_library().akm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        r, g = _i(), _i()
+        waves = np.zeros(64, np.int32)
+        _check(load_library().akm_batch_info(self._h, ctypes.byref(r), ctypes.byref(g),
+                                             waves.ctypes.data))
+        return {"restart": r.value, "groups": g.value, "group_waves": waves[:g.value].tolist()}
+
+    def solve(self, xs, bs, max_cycles=-1, tol=1e-10):
+        """Solves in place (xs: float64 CUDA tensors, initial guesses); returns
+        (relative residuals, operator applications, converged flags)."""
+        if len(xs) != self.count or len(bs) != self.count:
+            raise AkmInvalid(AKM_ERR_INVALID, "one x and one b per system")
+        for v in list(xs) + list(bs):
+            _vec_ptr(v, np.float64)
+        px = (ctypes.c_void_p * self.count)(*[v.data_ptr() for v in xs])
+        pb = (ctypes.c_void_p * self.count)(*[v.data_ptr() for v in bs])
+        res = np.zeros(self.count)
+        its = np.zeros(self.count, np.int32)
+        conv = np.zeros(self.count, np.int32)
+        _check(load_library().akm_batch_solve(self._h, px, pb, int(max_cycles), float(tol),
+                                              res.ctypes.data, its.ctypes.data,
+                                              conv.ctypes.data), allow_not_converged=True)
+        return res, its, conv.astype(bool)

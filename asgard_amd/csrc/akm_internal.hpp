@@ -116,4 +116,11 @@ int launch_kronmult(int D, int n, int mode, kron_args<P> const &a, int num_rows,
 template<typename P>
 int launch_scale(int64_t size, P beta, P *y, P gamma, const P *xg, hipStream_t stream);
 
+struct batch_system;
+//! Batched sparse-grid FP64 kronmult over a (system, row) work list of
+//! systems with the same (D, n); `grid` persistent workgroups.
+int launch_rows_batch(int D, int n, int waves, const kron_args<double> *args,
+                      const batch_system *sys, const int2 *work, int nwork, int grid,
+                      const int *jcol, const int *active, int phase, hipStream_t stream);
+
 } // namespace akm

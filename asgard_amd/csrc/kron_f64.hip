@@ -16,3 +16,14 @@ int launch_scale<double>(int64_t size, double beta, double *y, double gamma, con
   return launch_scale_impl<double>(size, beta, y, gamma, xg, stream);
 }
 } // namespace akm
+
+namespace akm
+{
+int launch_rows_batch(int D, int n, int waves, const kron_args<double> *args,
+                      const batch_system *sys, const int2 *work, int nwork, int grid,
+                      const int *jcol, const int *active, int phase, hipStream_t stream)
+{
+  return launch_rows_batch_impl(D, n, waves, args, sys, work, nwork, grid, jcol, active, phase,
+                                stream);
+}
+} // namespace akm

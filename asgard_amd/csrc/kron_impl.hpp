@@ -161,9 +161,10 @@ __device__ __forceinline__ void row_range(kron_args<P> const &a, int row, int64_
 #define AKM_WAVES_PER_EU 1
 #endif
 
+//! One output row of the register kernel (the whole workgroup works on it);
+//! `red` is the LDS buffer of the final reduction (CPW * (N + 1) entries).
 template<typename P, int D, int n, int g, int MODE, int W>
-__global__ void __launch_bounds__(W *wave_size)
-    __attribute__((amdgpu_waves_per_eu(AKM_WAVES_PER_EU))) kron_rows(kron_args<P> a)
+__device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P *red)
 {
   constexpr int N = ipow<n, D>::value;
   constexpr int G = ipow<n, g>::value;
@@ -174,8 +175,6 @@ __global__ void __launch_bounds__(W *wave_size)
   constexpr int EPT = (N + W * wave_size - 1) / (W * wave_size);
   static_assert(G <= wave_size, "lane group larger than a wavefront");
 
-  __shared__ P red[CPW * LDN];
-
   const int tid = threadIdx.x;
   const int wave = tid / wave_size;
   const int lane = tid - wave * wave_size;
@@ -183,7 +182,6 @@ __global__ void __launch_bounds__(W *wave_size)
   const int sub = lane - slot * G;
   const bool lane_used = slot < CPW;
 
-  const int row = a.row_order[blockIdx.x];
   int64_t jbeg, jend;
   row_range<P, MODE>(a, row, jbeg, jend);
 
@@ -298,6 +296,63 @@ __global__ void __launch_bounds__(W *wave_size)
     const int e = tid + i * W * wave_size;
     if (e < N)
       store_row(a, ybase + e, tot[i]);
+  }
+}
+
+template<typename P, int D, int n, int g, int MODE, int W>
+__global__ void __launch_bounds__(W *wave_size)
+    __attribute__((amdgpu_waves_per_eu(AKM_WAVES_PER_EU))) kron_rows(kron_args<P> a)
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int CPW = wave_size / ipow<n, g>::value;
+  __shared__ P red[CPW * (N + 1)];
+  kron_row_body<P, D, n, g, MODE, W>(a, a.row_order[blockIdx.x], red);
+}
+
+// -------------------------------------------------- batched row kernel ---
+
+//! Persistent kernel over the (system, row) work list of a batch of
+//! sparse-grid operators with the same (d, n): workgroup b takes items b,
+//! b + gridDim.x, ...  Phase 0 forms the residual of every active system,
+//! V_s[0] = b_s - (I - dt_s A_s) x_s (V_s[0] holds b_s on entry); phase 1
+//! applies the implicit operator to the current Krylov column j = *jcol,
+//! V_s[j+1] = (I - dt_s A_s) V_s[j].
+template<int D, int n, int g, int W>
+__global__ void __launch_bounds__(W *wave_size)
+kron_rows_batch(const kron_args<double> *args, const batch_system *sys, const int2 *work,
+                int nwork, const int *jcol, const int *active, int phase)
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int CPW = wave_size / ipow<n, g>::value;
+  __shared__ double red[CPW * (N + 1)];
+  for (int item = blockIdx.x; item < nwork; item += gridDim.x)
+  {
+    const int2 wk = work[item];
+    if (active[wk.x] == 0)
+      continue;
+    kron_args<double> a = args[wk.x];
+    const batch_system v = sys[wk.x];
+    if (phase == 0)
+    {
+      a.x = v.X;
+      a.y = v.V;
+      a.xg = v.X;
+      a.alpha = v.dt;
+      a.beta = 1.0;
+      a.gamma = -1.0;
+    }
+    else
+    {
+      const double *xj = v.V + int64_t(*jcol) * v.n;
+      a.x = xj;
+      a.y = v.V + int64_t(*jcol + 1) * v.n;
+      a.xg = xj;
+      a.alpha = -v.dt;
+      a.beta = 0.0;
+      a.gamma = 1.0;
+    }
+    kron_row_body<double, D, n, g, mode_grid, W>(a, wk.y, red);
+    __syncthreads();
   }
 }
 
@@ -774,6 +829,62 @@ int launch_kronmult_impl(int D, int n, int mode, kron_args<P> const &a, int num_
       kron_generic<P, mode_dense><<<num_rows, generic_threads, lds, stream>>>(a, D, n);
   }
   return check_launch("kronmult");
+}
+
+template<int D, int n, int W>
+bool launch_rows_batch_w(const kron_args<double> *args, const batch_system *sys, const int2 *work,
+                         int nwork, int grid, const int *jcol, const int *active, int phase,
+                         hipStream_t stream)
+{
+  constexpr int g = lead_dims<double>(D, n);
+  if constexpr (ipow<n, g>::value > wave_size)
+  {
+    return false;
+  }
+  else
+  {
+    kron_rows_batch<D, n, g, W><<<grid, W * wave_size, 0, stream>>>(args, sys, work, nwork, jcol,
+                                                                    active, phase);
+    return true;
+  }
+}
+
+template<int D, int n>
+bool launch_rows_batch_dn(int waves, const kron_args<double> *args, const batch_system *sys,
+                          const int2 *work, int nwork, int grid, const int *jcol,
+                          const int *active, int phase, hipStream_t stream)
+{
+  switch (waves)
+  {
+  case 1: return launch_rows_batch_w<D, n, 1>(args, sys, work, nwork, grid, jcol, active, phase, stream);
+  case 2: return launch_rows_batch_w<D, n, 2>(args, sys, work, nwork, grid, jcol, active, phase, stream);
+  case 8: return launch_rows_batch_w<D, n, 8>(args, sys, work, nwork, grid, jcol, active, phase, stream);
+  default: return launch_rows_batch_w<D, n, 4>(args, sys, work, nwork, grid, jcol, active, phase, stream);
+  }
+}
+
+inline int launch_rows_batch_impl(int D, int n, int waves, const kron_args<double> *args,
+                                  const batch_system *sys, const int2 *work, int nwork, int grid,
+                                  const int *jcol, const int *active, int phase,
+                                  hipStream_t stream)
+{
+  if (nwork == 0)
+    return AKM_SUCCESS;
+  bool done = false;
+#define AKM_BCASE(DD, NN)                                                                          \
+  if (!done && D == DD && n == NN)                                                                 \
+    done = launch_rows_batch_dn<DD, NN>(waves, args, sys, work, nwork, grid, jcol, active, phase,  \
+                                        stream);
+  AKM_BCASE(1, 2) AKM_BCASE(1, 3) AKM_BCASE(1, 4)
+  AKM_BCASE(2, 2) AKM_BCASE(2, 3) AKM_BCASE(2, 4)
+  AKM_BCASE(3, 2) AKM_BCASE(3, 3) AKM_BCASE(3, 4)
+  AKM_BCASE(4, 2) AKM_BCASE(4, 3)
+  AKM_BCASE(5, 2) AKM_BCASE(5, 3)
+  AKM_BCASE(6, 2)
+#undef AKM_BCASE
+  if (!done)
+    return set_error(AKM_ERR_UNSUPPORTED, "batched kronmult: (d, n) = (%d, %d) not supported", D, n);
+  return check_launch("batched kronmult");
 }
 
 template<typename P>

@@ -83,6 +83,16 @@ struct kron_args
   P alpha, beta, gamma;
 };
 
+//! Vectors of one system of a batched solve: the Krylov basis V (restart + 1
+//! columns of n), the solution X, the size and the time step.
+struct batch_system
+{
+  double *V;
+  double *X;
+  int64_t n;
+  double dt;
+};
+
 //! Padded n*n block size of the re-laid-out factor array (16-byte rows).
 template<typename P>
 __host__ __device__ constexpr int padded_block(int n)

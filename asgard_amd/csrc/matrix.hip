@@ -117,11 +117,8 @@ static int lds_cap_bytes(akm_matrix_s const &m)
 }
 
 template<typename P>
-static int apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gamma,
-                       const P *xg)
+static kron_args<P> matrix_args(akm_matrix_s const &m)
 {
-  if (m.T == 0 || alpha == P{0} || m.rows == 0)
-    return launch_scale<P>(m.output_size(), beta, y, gamma, xg, m.stream);
   kron_args<P> a{};
   a.vA = reinterpret_cast<const P *>(m.vA.data());
   a.iA = m.iA.data();
@@ -144,6 +141,18 @@ static int apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gam
   a.max_table_elems = m.max_table_elems;
   a.lds_cap = (m.vAk.data() != nullptr) ? lds_cap_bytes(m) : 0;
   a.row_order = m.row_order.data();
+  return a;
+}
+
+kron_args<double> kron_args_f64(akm_matrix_s const &m) { return matrix_args<double>(m); }
+
+template<typename P>
+static int apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gamma,
+                       const P *xg)
+{
+  if (m.T == 0 || alpha == P{0} || m.rows == 0)
+    return launch_scale<P>(m.output_size(), beta, y, gamma, xg, m.stream);
+  kron_args<P> a = matrix_args<P>(m);
   a.x = x;
   a.y = y;
   a.xg = xg;

@@ -68,4 +68,7 @@ int apply_device(akm_matrix_s &m, double alpha, const void *x, double beta, void
 
 //! Wavefronts per row workgroup used for the matrix (explicit or automatic).
 int waves_per_row(akm_matrix_s const &m);
+
+//! Kernel arguments of an FP64 matrix (vectors and scalars left empty).
+kron_args<double> kron_args_f64(akm_matrix_s const &m);
 } // namespace akm

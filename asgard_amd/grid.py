@@ -61,6 +61,56 @@ def element_table(levels, max_level=None, full_grid=False):
     return table, ids
 
 
+def children_1d(level, pos):
+    """Children of a 1D hierarchical cell: level 0 -> (1, 0); level 1 -> (2, 0),
+    (2, 1); level l >= 2 -> (l + 1, 2p), (l + 1, 2p + 1)."""
+    if level == 0:
+        return [(1, 0)]
+    if level == 1:
+        return [(2, 0), (2, 1)]
+    return [(level + 1, 2 * pos), (level + 1, 2 * pos + 1)]
+
+
+def refine(table, fraction, seed=0, max_level=None):
+    """Adaptive refinement of a cell table, as an input generator: the leaf
+    cells (no child present in any dimension) are refined with probability
+    ``fraction`` (drawn from numpy default_rng(seed)); a refined cell gets its
+    children in every dimension (one level up in that dimension, the others
+    unchanged), appended after the existing cells in order of discovery.
+    ``max_level`` caps the level of new cells.  Returns the new table."""
+    table = np.asarray(table, dtype=np.int32)
+    cells, twod = table.shape
+    d = twod // 2
+    present = {tuple(r) for r in table.tolist()}
+
+    def kids(cell):
+        out = []
+        for k in range(d):
+            for (l, p) in children_1d(cell[k], cell[d + k]):
+                if max_level is not None and l > max_level:
+                    continue
+                c = list(cell)
+                c[k], c[d + k] = l, p
+                out.append(tuple(c))
+        return out
+
+    rng = np.random.default_rng(seed)
+    rows = table.tolist()
+    leaves = [tuple(r) for r in rows if not any(c in present for c in kids(tuple(r)))]
+    pick = rng.random(len(leaves)) < fraction
+    new = []
+    for leaf, chosen in zip(leaves, pick):
+        if not chosen:
+            continue
+        for c in kids(leaf):
+            if c not in present:
+                present.add(c)
+                new.append(c)
+    if not new:
+        return table.copy()
+    return np.concatenate([table, np.array(new, dtype=np.int32).reshape(-1, twod)])
+
+
 def cell_1d_index(table):
     table = np.asarray(table)
     d = table.shape[1] // 2

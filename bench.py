@@ -201,6 +201,83 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
     return out
 
 
+def batch_systems(count=64, seed=0):
+    """The batched-solve workload: `count` implicit steps, d ~ U{2..5}, n in
+    {2, 3}, each on the |l|_1 <= 5 sparse grid with 10-30% of its leaf cells
+    refined one level (seeded), d terms (advection in the first d/2
+    dimensions, diffusion nu = 0.1 in the others), dt = 1e-3, b ~ U[-1, 1]."""
+    import asgard_amd
+    from asgard_amd import dg, grid
+
+    rng = np.random.default_rng(seed)
+    mats, dts, bs = [], [], []
+    for q in range(count):
+        d = int(rng.integers(2, 6))
+        n = int(rng.choice([2, 3]))
+        frac = float(rng.uniform(0.1, 0.3))
+        table, _ = grid.element_table([5] * d)
+        table = grid.refine(table, frac, seed=seed * 1000 + q)
+        L = int(table[:, :d].max())
+        ops = dg.operators_advection_diffusion(d, n, L, n_adv=max(1, d // 2), nu=0.1)
+        vA = grid.build_vA(d, n, d, L, ops)
+        mats.append(asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, d, vA))
+        dts.append(1e-3)
+        bs.append(rng.uniform(-1.0, 1.0, table.shape[0] * n ** d))
+    return mats, dts, bs
+
+
+def batch_comparison(count=64, restart=50):
+    """Aggregate solves per second of `count` implicit steps: one at a time
+    with the device GMRES (the reference algorithm, absolute tolerance
+    1e-10*|b|) against all at once with the batched solver (true FP64
+    residual <= 1e-10*|b|).  Both start from x0 = b; the batch is built and
+    solved once untimed (graph instantiation, first launches) first."""
+    import torch
+
+    import asgard_amd
+
+    mats, dts, bs_host = batch_systems(count)
+    bs = [torch.from_numpy(b).cuda() for b in bs_host]
+    for m in mats:
+        m.set_stream(torch.cuda.current_stream().cuda_stream)
+    sizes = [m.output_size() for m in mats]
+    out = {"workload": f"{count} implicit steps (I - dt A_s) x_s = b_s, d ~ U{{2..5}}, n in {{2,3}}, "
+                       "|l|_1<=5 sparse grids with 10-30% of leaf cells refined one level, d terms "
+                       "(advection + diffusion nu=0.1), dt=1e-3, b ~ U[-1,1], GMRES(50), "
+                       "tol 1e-10 relative",
+           "systems": count, "unknowns_total": int(sum(sizes)), "unknowns_max": int(max(sizes)),
+           "pairs_total": int(sum(m.num_pairs() for m in mats))}
+
+    def sequential():
+        xs = [b.clone() for b in bs]
+        its = 0
+        for m, dt, x, b in zip(mats, dts, xs, bs):
+            info = asgard_amd.simple_gmres_euler(dt, m, x, b, restart, -1,
+                                                 1e-10 * float(torch.linalg.norm(b).item()))
+            its += info.iterations
+        return xs, its
+
+    sequential()  # workspaces
+    (xs_seq, its_seq), t_seq = timed(sequential)
+    batch = asgard_amd.BatchSolver(mats, dts, restart=restart)
+    warm = [b.clone() for b in bs]
+    batch.solve(warm, bs)
+    xs = [b.clone() for b in bs]
+    (res, its, conv), t_batch = timed(lambda: batch.solve(xs, bs))
+    diff = max(float(torch.linalg.norm(a - c).item() / torch.linalg.norm(c).item())
+               for a, c in zip(xs, xs_seq))
+    out.update({
+        "sequential_fp64": {"seconds": round(t_seq, 4), "solves_per_s": round(count / t_seq, 2),
+                            "applies": int(its_seq)},
+        "batched_fp64": {"seconds": round(t_batch, 4), "solves_per_s": round(count / t_batch, 2),
+                         "applies": int(its.sum()), "converged": int(conv.sum()),
+                         "max_rel_residual": float(res.max()), **batch.info()},
+        "speedup": round(t_seq / t_batch, 3),
+        "max_rel_diff_batched_vs_sequential": diff,
+    })
+    return out
+
+
 def solve_cpu_baseline(d, n, L, T, table, vA, dt, b, restart, x_gpu):
     """The oracle's restatement of simple_gmres_euler with its OpenMP kronmult
     on the host cores, same system, same tolerance; and the GPU solution's
@@ -243,9 +320,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 512 MiB device copy (HBM counter calibration)")
-    ap.add_argument("--solve", default="d4n2,d6n2",
+    ap.add_argument("--solve", default="d4n2,d6n2,batch64",
                     help="implicit-step solve comparisons to run (comma list of "
-                         + ", ".join(SOLVE_WORKLOADS) + "; 'none' to skip)")
+                         + ", ".join(list(SOLVE_WORKLOADS) + ["batch64"]) + "; 'none' to skip)")
     ap.add_argument("--solve-only", action="store_true",
                     help="print only the solve comparisons (one JSON line each)")
     ap.add_argument("--samples", type=int, default=200_000,
@@ -255,11 +332,14 @@ def main():
         calibrate_copy()
         return
     solves = [] if args.solve == "none" else [k for k in args.solve.split(",") if k]
+    def run_solve(k):
+        if k == "batch64":
+            return batch_comparison(64)
+        return solve_comparison(k, samples=args.samples, with_cpu=not args.no_cpu)
+
     if args.solve_only:
         for k in solves:
-            print(json.dumps({"solve": k, **solve_comparison(k, samples=args.samples,
-                                                            with_cpu=not args.no_cpu)}),
-                  flush=True)
+            print(json.dumps({"solve": k, **run_solve(k)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
@@ -340,8 +420,7 @@ def main():
         line["solve"] = {}
         for k in solves:
             try:
-                line["solve"][k] = solve_comparison(k, samples=args.samples,
-                                                    with_cpu=not args.no_cpu)
+                line["solve"][k] = run_solve(k)
             except Exception as e:  # report, keep the kronmult line
                 line["solve"][k] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line))

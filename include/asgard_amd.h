@@ -222,6 +222,41 @@ akm_status akm_gmres_euler_mixed(akm_matrix mat, akm_matrix mat32, akm_precond p
                                  double *x, const double *b, int restart, int max_cycles,
                                  double tol, double inner_tol, akm_solve_stats *stats);
 
+/* --------------------------------------------------- batched solves -- */
+/* No counterpart in the reference, which solves one implicit step at a time
+ * (time_advance::implicit_advance, src/time_advance.cpp:432-452, one
+ * simple_gmres_euler each).  A batch of independent implicit steps
+ * (I - dt_s A_s) x_s = b_s over FP64 sparse-grid matrices of any sizes and
+ * (d, n) (n in 2..4; d <= 6 with n = 2, d <= 5 with n = 3, d <= 3 with
+ * n = 4): restarted GMRES(restart) in lockstep, every operator apply one
+ * persistent launch per (d, n) group over a (system, row) work list, every
+ * restart cycle three hipGraphs (start, Arnoldi iteration, update).  The
+ * matrices must outlive the batch and keep their coefficients. */
+typedef struct akm_batch_s *akm_batch;
+
+/* Builds the batch: work lists, device workspaces, per-group workgroup shape
+ * chosen by timing (1, 2, 4 or 8 wavefronts per row), captured graphs.
+ * restart <= 0 selects 50 (at most 255 and the smallest system size). */
+akm_status akm_batch_create(int count, const akm_matrix *mats, const double *dts, int restart,
+                            akm_batch *batch);
+
+/* restart used, number of (d, n) groups and each group's wavefronts per row
+ * (group_waves has num_groups entries; groups ordered by (d, n)). */
+akm_status akm_batch_info(akm_batch batch, int *restart, int *num_groups, int *group_waves);
+
+/* Solves every system to |b_s - (I - dt_s A_s) x_s| <= tol*|b_s| (true FP64
+ * residual, tol <= 0 selects 1e-10), at most max_cycles restart cycles
+ * (<= 0: 1000).  x[s] (initial guess in, solution out) and rhs[s] are FP64
+ * device vectors; the call synchronizes the device first and returns with the
+ * solutions in place.  Per-system outputs (host arrays of count, may be NULL):
+ * relative residual, operator applications, converged flag.  Returns
+ * AKM_ERR_NOT_CONVERGED if any system ran out of cycles. */
+akm_status akm_batch_solve(akm_batch batch, double *const *x, const double *const *rhs,
+                           int max_cycles, double tol, double *rel_residual, int *iterations,
+                           int *converged);
+
+akm_status akm_batch_destroy(akm_batch batch);
+
 /* ------------------------------------------- learned preconditioner -- */
 /* No counterpart in the reference.  A per-cell MLP right preconditioner:
  * for cell c with residual block r_c (n^d values) and s = |r_c| it reads

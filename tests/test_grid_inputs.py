@@ -48,3 +48,36 @@ def test_dense_terms_match_oracle(d, n, T, L):
 def test_cell_1d_index_matches_oracle():
     t, _ = grid.element_table((4, 3, 5))
     assert np.array_equal(grid.cell_1d_index(t), oracle.cell_1d_index(t))
+
+
+def test_refine_adds_children_of_chosen_leaves():
+    t, _ = grid.element_table([4, 4])
+    r = grid.refine(t, 0.3, seed=2)
+    assert np.array_equal(r[:len(t)], t)  # existing cells keep their order
+    cells = {tuple(c) for c in r.tolist()}
+    assert len(cells) == len(r)  # no duplicates
+    new = r[len(t):]
+    assert len(new) > 0 and new[:, :2].max() == 5
+    base = {tuple(c) for c in t.tolist()}
+    for c in new.tolist():
+        # every new cell is a child (one dimension one level up) of an old cell
+        parents = []
+        for k in range(2):
+            lev, pos = c[k], c[2 + k]
+            for pl in range(lev):
+                for pp in range(max(1, 1 << max(pl - 1, 0))):
+                    if (pl, pp) != (lev, pos) and (lev, pos) in grid.children_1d(pl, pp):
+                        q = list(c)
+                        q[k], q[2 + k] = pl, pp
+                        parents.append(tuple(q))
+        assert any(p in base for p in parents), c
+
+
+def test_refine_fraction_limits():
+    t, _ = grid.element_table([3, 3, 3])
+    assert np.array_equal(grid.refine(t, 0.0, seed=0), t)
+    full = grid.refine(t, 1.0, seed=0)
+    again = grid.refine(full, 0.0, seed=0)
+    assert np.array_equal(full, again)
+    capped = grid.refine(t, 1.0, seed=0, max_level=3)
+    assert capped[:, :3].max() <= 3
