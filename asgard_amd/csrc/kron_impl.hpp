@@ -410,6 +410,23 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
       base += (a.conn_pntr[rr[k] + 1] - cb[k]) * T * nnp;
     }
   }
+  // the first column's loads go out before the tables are staged, so their
+  // latency overlaps the staging
+  const int64_t jstep = int64_t(W) * CPW;
+  int64_t j = jbeg + int64_t(wave) * CPW + slot;
+  bool valid = lane_used && j < jend;
+  P X[R];
+  unsigned long long cp = 0;
+  if (valid)
+  {
+    const int col = a.indx[j];
+    cp = a.epack[col];
+    const P *xc = a.x + int64_t(col) * N + sub * R;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      X[r] = xc[r];
+  }
+
   for (int e = tid; e < D * M; e += nthreads)
   {
     const int k = e / M, c = e - k * M;
@@ -437,21 +454,6 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
 #pragma unroll
   for (int r = 0; r < R; r++)
     acc[r] = 0;
-
-  const int64_t jstep = int64_t(W) * CPW;
-  int64_t j = jbeg + int64_t(wave) * CPW + slot;
-  bool valid = lane_used && j < jend;
-  P X[R];
-  unsigned long long cp = 0;
-  if (valid)
-  {
-    const int col = a.indx[j];
-    cp = a.epack[col];
-    const P *xc = a.x + int64_t(col) * N + sub * R;
-#pragma unroll
-    for (int r = 0; r < R; r++)
-      X[r] = xc[r];
-  }
   for (int64_t j0 = jbeg + int64_t(wave) * CPW; j0 < jend; j0 += jstep)
   {
     const int64_t jn = j + jstep;

@@ -49,15 +49,20 @@ template<int EP>
 __global__ void __launch_bounds__(mm_threads)
 mlp_gemm(int M, int N, int K, const float *A, int64_t sam, int64_t sak, const float *B,
          int64_t sbk, int64_t sbn, float *C, int64_t ldc, const float *bias, const float *mask,
-         int64_t ldm)
+         int64_t ldm, int kchunk, int64_t zstride)
 {
   __shared__ float As[mm_k][mm_tile + 4];
   __shared__ float Bs[mm_k][mm_tile + 4];
   const int t = threadIdx.x;
   const int m0 = blockIdx.y * mm_tile, c0 = blockIdx.x * mm_tile;
   const int tm = (t / 16) * 4, tc = (t % 16) * 4;
+  // split K: block z sums k in [z*kchunk, min(K, (z+1)*kchunk)) into its own
+  // output slice C + z*zstride
+  const int kb = blockIdx.z * kchunk;
+  const int ke = min(K, kb + kchunk);
+  C += blockIdx.z * zstride;
   float acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += mm_k)
+  for (int k0 = kb; k0 < ke; k0 += mm_k)
   {
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -65,9 +70,9 @@ mlp_gemm(int M, int N, int K, const float *A, int64_t sam, int64_t sak, const fl
       const int e = t + mm_threads * q; // 0 .. 1023
       const int mm = e % mm_tile, kk = e / mm_tile;
       const int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? A[gm * sam + gk * sak] : 0.0f;
+      As[kk][mm] = (gm < M && gk < ke) ? A[gm * sam + gk * sak] : 0.0f;
       const int gc = c0 + mm;
-      Bs[kk][mm] = (gc < N && gk < K) ? B[gk * sbk + gc * sbn] : 0.0f;
+      Bs[kk][mm] = (gc < N && gk < ke) ? B[gk * sbk + gc * sbn] : 0.0f;
     }
     __syncthreads();
 #pragma unroll
@@ -165,17 +170,34 @@ __global__ void __launch_bounds__(256) mlp_sum(int count, const double *partial,
     out[0] = accumulate ? out[0] + sh[0] : sh[0];
 }
 
-//! out[c] = sum over rows of X[r*ld + c], c < cols.
+//! partial[z*cols + c] = sum over rows r in chunk z of X[r*ld + c]
+//! (block (x, z): 256 columns of chunk z).
 __global__ void __launch_bounds__(256)
-mlp_colsum(int64_t rows, int cols, const float *X, int64_t ld, float *out)
+mlp_colsum_part(int64_t rows, int cols, const float *X, int64_t ld, int64_t chunk,
+                float *partial)
 {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= cols)
     return;
+  const int64_t r0 = blockIdx.z * chunk;
+  const int64_t r1 = min(rows, r0 + chunk);
   float s = 0.0f;
-  for (int64_t r = 0; r < rows; r++)
+  for (int64_t r = r0; r < r1; r++)
     s += X[r * ld + c];
-  out[c] = s;
+  partial[int64_t(blockIdx.z) * cols + c] = s;
+}
+
+//! out[e] = sum over z < parts of partial[z*count + e] (fixed order).
+__global__ void __launch_bounds__(256)
+mlp_sum_parts(int64_t count, int parts, const float *partial, float *out)
+{
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < count; e += 256ll * gridDim.x)
+  {
+    float s = 0.0f;
+    for (int z = 0; z < parts; z++)
+      s += partial[z * count + e];
+    out[e] = s;
+  }
 }
 
 //! Adam (Kingma & Ba, 2015): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
@@ -405,6 +427,8 @@ struct akm_mlp_s final : akm_precond_s
   // batch buffers (sized for `batch` rows)
   int batch = 0;
   dev_buffer<float> Xb, Tb, H1, H2, O, dO, dH2, dH1;
+  dev_buffer<float> scratch; // split-K partial products
+  static constexpr int split_parts = 16;
   dev_buffer<double> partial, loss;
   hipStream_t stream = nullptr;
 
@@ -425,6 +449,8 @@ struct akm_mlp_s final : akm_precond_s
     AKM_TRY(dH1.allocate(int64_t(B) * H));
     AKM_TRY(partial.allocate(4096));
     AKM_TRY(loss.allocate(2));
+    const int64_t widest = std::max<int64_t>({int64_t(H) * K1, int64_t(H) * H, int64_t(N3) * H});
+    AKM_TRY(scratch.allocate(split_parts * widest));
     return AKM_SUCCESS;
   }
 
@@ -433,10 +459,37 @@ struct akm_mlp_s final : akm_precond_s
            const float *B, int64_t sbk, int64_t sbn, float *C, int64_t ldc,
            const float *bias = nullptr, const float *mask = nullptr, int64_t ldm = 0)
   {
-    dim3 grid((Nc + mm_tile - 1) / mm_tile, (M + mm_tile - 1) / mm_tile);
+    dim3 grid((Nc + mm_tile - 1) / mm_tile, (M + mm_tile - 1) / mm_tile, 1);
     mlp_gemm<EP><<<grid, mm_threads, 0, s>>>(M, Nc, K, A, sam, sak, B, sbk, sbn, C, ldc, bias,
-                                              mask, ldm);
+                                              mask, ldm, K, 0);
     return check_launch("mlp gemm");
+  }
+
+  //! C (M x Nc, ldc = Nc) = A B for a long K (the batch): K is split over
+  //! blocks into partial products that are then summed in a fixed order.
+  int gemm_long_k(hipStream_t s, int M, int Nc, int K, const float *A, int64_t sam, int64_t sak,
+                  const float *B, int64_t sbk, int64_t sbn, float *C)
+  {
+    const int parts = std::max(1, std::min(split_parts, (K + 127) / 128));
+    const int kchunk = (K + parts - 1) / parts;
+    const int64_t zstride = int64_t(M) * Nc;
+    dim3 grid((Nc + mm_tile - 1) / mm_tile, (M + mm_tile - 1) / mm_tile, parts);
+    mlp_gemm<ep_store><<<grid, mm_threads, 0, s>>>(M, Nc, K, A, sam, sak, B, sbk, sbn,
+                                                    scratch.data(), Nc, nullptr, nullptr, 0,
+                                                    kchunk, zstride);
+    mlp_sum_parts<<<grid_for(zstride), 256, 0, s>>>(zstride, parts, scratch.data(), C);
+    return check_launch("mlp gemm (split k)");
+  }
+
+  //! out[c] = sum over the B rows of X[r*ld + c].
+  int colsum(hipStream_t s, int64_t rows, int cols, const float *X, int64_t ld, float *out)
+  {
+    const int parts = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(split_parts, (rows + 63) / 64)));
+    const int64_t chunk = (rows + parts - 1) / parts;
+    mlp_colsum_part<<<dim3((cols + 255) / 256, 1, parts), 256, 0, s>>>(rows, cols, X, ld, chunk,
+                                                                      scratch.data());
+    mlp_sum_parts<<<grid_for(cols), 256, 0, s>>>(cols, parts, scratch.data(), out);
+    return check_launch("mlp column sums");
   }
 
   //! FP32 forward of B rows of Xb into H1, H2, O.
@@ -463,18 +516,18 @@ struct akm_mlp_s final : akm_precond_s
     mlp_sum<<<1, 256, 0, s>>>(blocks, partial.data(), loss.data(), accumulate);
     AKM_TRY(check_launch("mlp loss"));
     // layer 3
-    AKM_TRY(gemm<ep_store>(s, N3, H, B, dO.data(), 1, N3, H2.data(), H, 1, g + oW3, H));
-    mlp_colsum<<<(N3 + 255) / 256, 256, 0, s>>>(B, N3, dO.data(), N3, g + ob3);
+    AKM_TRY(gemm_long_k(s, N3, H, B, dO.data(), 1, N3, H2.data(), H, 1, g + oW3));
+    AKM_TRY(colsum(s, B, N3, dO.data(), N3, g + ob3));
     AKM_TRY(gemm<ep_mask>(s, B, H, N3, dO.data(), N3, 1, p + oW3, H, 1, dH2.data(), H, nullptr,
                           H2.data(), H));
     // layer 2
-    AKM_TRY(gemm<ep_store>(s, H, H, B, dH2.data(), 1, H, H1.data(), H, 1, g + oW2, H));
-    mlp_colsum<<<(H + 255) / 256, 256, 0, s>>>(B, H, dH2.data(), H, g + ob2);
+    AKM_TRY(gemm_long_k(s, H, H, B, dH2.data(), 1, H, H1.data(), H, 1, g + oW2));
+    AKM_TRY(colsum(s, B, H, dH2.data(), H, g + ob2));
     AKM_TRY(gemm<ep_mask>(s, B, H, H, dH2.data(), H, 1, p + oW2, H, 1, dH1.data(), H, nullptr,
                           H1.data(), H));
     // layer 1
-    AKM_TRY(gemm<ep_store>(s, H, K1, B, dH1.data(), 1, H, X, K1, 1, g + oW1, K1));
-    mlp_colsum<<<(H + 255) / 256, 256, 0, s>>>(B, H, dH1.data(), H, g + ob1);
+    AKM_TRY(gemm_long_k(s, H, K1, B, dH1.data(), 1, H, X, K1, 1, g + oW1));
+    AKM_TRY(colsum(s, B, H, dH1.data(), H, g + ob1));
     return check_launch("mlp backward");
   }
 
