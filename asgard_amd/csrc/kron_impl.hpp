@@ -138,6 +138,71 @@ __device__ __forceinline__ void contract_term(const P (&X)[R], P (&w)[R], P (&ac
   }
 }
 
+//! Lane-spread dimensions are exchanged through LDS (one transpose per
+//! (column, term)) rather than by lane reads when n >= 3: a lane read moves
+//! one value per instruction and n of them are needed per entry and stage.
+template<int n, int g>
+constexpr bool transposed_exchange()
+{
+  return g > 0 && n != 2;
+}
+
+//! Orders the LDS traffic of one wavefront around a cross-lane exchange.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+//! Stages K .. 0 in the transposed layout, where the registers hold the
+//! leading D - g dimensions (so dims 0 .. g-1 are local); the last one
+//! (K == 0) accumulates into acc.
+template<typename P, int D, int n, int g, int R, int K>
+__device__ __forceinline__ void transposed_stages(P (&w)[R], P (&acc)[R],
+                                                  const P *const (&A)[D])
+{
+  if constexpr (K >= 0)
+  {
+    constexpr int inner = ipow<n, D - g - 1 - K>::value;
+    if constexpr (K == 0)
+      stage_into<P, n, R, inner, true>(w, acc, A[0]);
+    else
+      stage_into<P, n, R, inner, false>(w, w, A[K]);
+    transposed_stages<P, D, n, g, R, K - 1>(w, acc, A);
+  }
+}
+
+//! Full contraction of one (column, term) into acc with the exchange done
+//! as a transpose through the lane group's LDS slice tb (N entries): before
+//! it lane `sub` holds entries sub*R + r (lane-spread dims leading), after
+//! it entries r*G + sub (lane-spread dims trailing), which is also the
+//! layout of acc.
+template<typename P, int D, int n, int g, int R, bool FROM_W>
+__device__ __forceinline__ void contract_term_t(const P (&X)[R], P (&w)[R], P (&acc)[R],
+                                                const P *const (&A)[D], int sub, P *tb)
+{
+  constexpr int G = ipow<n, g>::value;
+  static_assert(D - g >= g, "the transposed layout needs D - g >= g");
+  contract_local<P, D, n, g, R, D - 1, FROM_W>(X, w, acc, A);
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    tb[sub * R + r] = w[r];
+  wave_lds_sync();
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    w[r] = tb[r * G + sub];
+  wave_lds_sync();
+  transposed_stages<P, D, n, g, R, g - 1>(w, acc, A);
+}
+
+//! Position in the row tensor of accumulator entry r of lane `sub`.
+template<int N, int G, int R, bool TR>
+__device__ __forceinline__ int acc_index(int sub, int r)
+{
+  return TR ? r * G + sub : sub * R + r;
+}
+
 //! Column range [beg, end) of a row.
 template<typename P, int MODE>
 __device__ __forceinline__ void row_range(kron_args<P> const &a, int row, int64_t &beg,
@@ -164,7 +229,7 @@ __device__ __forceinline__ void row_range(kron_args<P> const &a, int row, int64_
 //! One output row of the register kernel (the whole workgroup works on it);
 //! `red` is the LDS buffer of the final reduction (CPW * (N + 1) entries).
 template<typename P, int D, int n, int g, int MODE, int W>
-__device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P *red)
+__device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P *red, P *tbuf)
 {
   constexpr int N = ipow<n, D>::value;
   constexpr int G = ipow<n, g>::value;
@@ -175,12 +240,14 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
   constexpr int EPT = (N + W * wave_size - 1) / (W * wave_size);
   static_assert(G <= wave_size, "lane group larger than a wavefront");
 
+  constexpr bool TR = transposed_exchange<n, g>();
   const int tid = threadIdx.x;
   const int wave = tid / wave_size;
   const int lane = tid - wave * wave_size;
   const int slot = lane / G;
   const int sub = lane - slot * G;
   const bool lane_used = slot < CPW;
+  P *tb = TR ? tbuf + (wave * CPW + slot) * N : nullptr;
 
   int64_t jbeg, jend;
   row_range<P, MODE>(a, row, jbeg, jend);
@@ -260,10 +327,15 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
             Ak[k] = a.vA + ia[k];
         }
         P w[R];
-        contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
+        if constexpr (TR)
+          contract_term_t<P, D, n, g, R, false>(X, w, acc, Ak, sub, tb);
+        else
+          contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
       }
     }
   }
+  if constexpr (TR)
+    __syncthreads(); // red may share LDS with the transpose slices
 
   // sum the column slots of every wavefront, in a fixed order
   P tot[EPT];
@@ -276,7 +348,7 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
     {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        red[slot * LDN + sub * R + r] = acc[r];
+        red[slot * LDN + acc_index<N, G, R, TR>(sub, r)] = acc[r];
     }
     __syncthreads();
 #pragma unroll
@@ -305,8 +377,46 @@ __global__ void __launch_bounds__(W *wave_size)
 {
   constexpr int N = ipow<n, D>::value;
   constexpr int CPW = wave_size / ipow<n, g>::value;
-  __shared__ P red[CPW * (N + 1)];
-  kron_row_body<P, D, n, g, MODE, W>(a, a.row_order[blockIdx.x], red);
+  if constexpr (transposed_exchange<n, g>())
+  {
+    // dynamic LDS: the wavefronts' transpose slices, then the reduction
+    extern __shared__ __align__(16) unsigned char rows_lds[];
+    P *buf = reinterpret_cast<P *>(rows_lds);
+    kron_row_body<P, D, n, g, MODE, W>(a, a.row_order[blockIdx.x], buf, buf);
+  }
+  else
+  {
+    __shared__ P red[CPW * (N + 1)];
+    kron_row_body<P, D, n, g, MODE, W>(a, a.row_order[blockIdx.x], red, nullptr);
+  }
+}
+
+//! Dynamic LDS bytes of kron_rows / kron_rows_batch.
+template<typename P, int D, int n, int g, int W>
+constexpr size_t rows_lds_bytes()
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int CPW = wave_size / ipow<n, g>::value;
+  constexpr size_t tr = size_t(W) * CPW * N, red = size_t(CPW) * (N + 1);
+  return transposed_exchange<n, g>() ? sizeof(P) * (tr > red ? tr : red) : 0;
+}
+
+//! Launches with `lds` dynamic bytes, raising the kernel's limit above
+//! 64 KB when needed; false if the device refuses.
+template<typename K>
+bool launch_with_lds(K kern, size_t lds)
+{
+  if (lds > 64 * 1024)
+  {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      return false;
+    }
+  }
+  return true;
 }
 
 // -------------------------------------------------- batched row kernel ---
@@ -324,7 +434,11 @@ kron_rows_batch(const kron_args<double> *args, const batch_system *sys, const in
 {
   constexpr int N = ipow<n, D>::value;
   constexpr int CPW = wave_size / ipow<n, g>::value;
-  __shared__ double red[CPW * (N + 1)];
+  constexpr bool TR = transposed_exchange<n, g>();
+  extern __shared__ __align__(16) unsigned char batch_lds[];
+  __shared__ double red_static[TR ? 1 : CPW * (N + 1)];
+  double *red = TR ? reinterpret_cast<double *>(batch_lds) : red_static;
+  double *tbuf = TR ? reinterpret_cast<double *>(batch_lds) : nullptr;
   for (int item = blockIdx.x; item < nwork; item += gridDim.x)
   {
     const int2 wk = work[item];
@@ -351,7 +465,7 @@ kron_rows_batch(const kron_args<double> *args, const batch_system *sys, const in
       a.beta = 0.0;
       a.gamma = 1.0;
     }
-    kron_row_body<double, D, n, g, mode_grid, W>(a, wk.y, red);
+    kron_row_body<double, D, n, g, mode_grid, W>(a, wk.y, red, tbuf);
     __syncthreads();
   }
 }
@@ -380,11 +494,15 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
   constexpr int nthreads = W * wave_size;
   static_assert(G <= wave_size, "lane group larger than a wavefront");
 
+  constexpr bool TR = transposed_exchange<n, g>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int M = a.M;
   int *loc = reinterpret_cast<int *>(smem_raw);
   P *fac = reinterpret_cast<P *>(smem_raw + ((D * M * sizeof(int) + 15) / 16) * 16);
   P *red = fac; // reused once the main loop is done
+  // transpose slices after the larger of the factor tables and the reduction
+  const int64_t fac_elems =
+      a.max_table_elems > int64_t(CPW) * (N + 1) ? a.max_table_elems : int64_t(CPW) * (N + 1);
 
   const int tid = threadIdx.x;
   const int wave = tid / wave_size;
@@ -393,6 +511,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
   const int sub = lane - slot * G;
   const bool lane_used = slot < CPW;
   const int T = a.T;
+  P *tb = TR ? fac + fac_elems + (wave * CPW + slot) * N : nullptr;
 
   const int row = a.row_order[blockIdx.x];
   const int64_t jbeg = a.pntr[row], jend = a.pntr[row + 1];
@@ -474,7 +593,10 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
       P w[R];
       for (int t = 0; t < T - 1; t++)
       {
-        contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
+        if constexpr (TR)
+          contract_term_t<P, D, n, g, R, false>(X, w, acc, Ak, sub, tb);
+        else
+          contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
 #pragma unroll
         for (int k = 0; k < D; k++)
           Ak[k] += nnp;
@@ -492,7 +614,10 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
         for (int r = 0; r < R; r++)
           X[r] = xn[r];
       }
-      contract_term<P, D, n, g, R, true>(X, w, acc, Ak, lane, sub);
+      if constexpr (TR)
+        contract_term_t<P, D, n, g, R, true>(X, w, acc, Ak, sub, tb);
+      else
+        contract_term<P, D, n, g, R, true>(X, w, acc, Ak, lane, sub);
       cp = cp_next;
     }
     j = jn;
@@ -510,7 +635,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
     {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        red[slot * LDN + sub * R + r] = acc[r];
+        red[slot * LDN + acc_index<N, G, R, TR>(sub, r)] = acc[r];
     }
     __syncthreads();
 #pragma unroll
@@ -702,7 +827,11 @@ bool launch_rows_w(kron_args<P> const &a, int num_rows, hipStream_t stream)
   }
   else
   {
-    kron_rows<P, D, n, g, MODE, W><<<num_rows, W * wave_size, 0, stream>>>(a);
+    constexpr size_t lds = rows_lds_bytes<P, D, n, g, W>();
+    auto kern = kron_rows<P, D, n, g, MODE, W>;
+    if (!launch_with_lds(kern, lds))
+      return false;
+    kern<<<num_rows, W * wave_size, lds, stream>>>(a);
     return true;
   }
 }
@@ -716,7 +845,8 @@ size_t grid_lds_bytes(kron_args<P> const &a)
   const size_t locb = ((size_t(D) * a.M * sizeof(int) + 15) / 16) * 16;
   const size_t tab = size_t(a.max_table_elems) * sizeof(P);
   const size_t redb = size_t(CPW) * (N + 1) * sizeof(P);
-  const size_t total = locb + (tab > redb ? tab : redb);
+  const size_t trb = transposed_exchange<n, g>() ? size_t(W) * CPW * N * sizeof(P) : 0;
+  const size_t total = locb + (tab > redb ? tab : redb) + trb;
   return (a.vAk != nullptr && total <= size_t(a.lds_cap)) ? total : 0;
 }
 
@@ -845,8 +975,11 @@ bool launch_rows_batch_w(const kron_args<double> *args, const batch_system *sys,
   }
   else
   {
-    kron_rows_batch<D, n, g, W><<<grid, W * wave_size, 0, stream>>>(args, sys, work, nwork, jcol,
-                                                                    active, phase);
+    constexpr size_t lds = rows_lds_bytes<double, D, n, g, W>();
+    auto kern = kron_rows_batch<D, n, g, W>;
+    if (!launch_with_lds(kern, lds))
+      return false;
+    kern<<<grid, W * wave_size, lds, stream>>>(args, sys, work, nwork, jcol, active, phase);
     return true;
   }
 }

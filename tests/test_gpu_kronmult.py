@@ -298,3 +298,29 @@ def test_time_apply_reports():
     ms = m.time_apply(1.0, x, 0.0, y, 5)
     assert ms > 0
     assert rel(host(y), oracle_apply(c, 1.0, 0.0)) < TOL64
+
+
+# ------------------------------------- lane-spread exchange through LDS ---
+
+@pytest.mark.parametrize("d,n,L,T", [(4, 3, 4, 2), (5, 3, 3, 2), (3, 4, 3, 2), (6, 3, 2, 1),
+                                     (4, 4, 2, 1)])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_transposed_exchange_parity(d, n, L, T, waves):
+    # n >= 3 with lane-spread dimensions: the exchange is a transpose through
+    # LDS and the accumulators live in the transposed layout; every kernel
+    # (register, LDS tables) and both precisions must still match the oracle
+    c = sparse_grid_case(d, n, L, T, seed=21)
+    ref = oracle_apply(c, 1.5, -0.5)
+    for dtype, tol in ((np.float64, TOL64), (np.float32, TOL32)):
+        m = asgard_amd.KronmultMatrix.sparse_grid(d, n, c["table"], L, T,
+                                                  c["vA"].astype(dtype), dtype=dtype)
+        for lds in (-1, 0):
+            m.set_tuning(waves_per_row=waves, lds_cap_bytes=lds)
+            y = dev(c["y"].astype(dtype))
+            m.apply(1.5, dev(c["x"].astype(dtype)), -0.5, y)
+            assert rel(host(y), ref) < tol, (dtype, lds)
+        mi = ia_matrix(c, dtype)
+        mi.set_tuning(waves_per_row=waves)
+        y = dev(c["y"].astype(dtype))
+        mi.apply(1.5, dev(c["x"].astype(dtype)), -0.5, y)
+        assert rel(host(y), ref) < tol, (dtype, "ia")
