@@ -809,7 +809,9 @@ constexpr int lead_dims(int D, int n)
   int g = 0, R = 1;
   for (int k = 0; k < D; k++)
     R *= n;
-  while (R > budget && g < D)
+  // n >= 3 exchanges by transpose, which needs at least as many register
+  // dimensions as lane-spread ones (D - g >= g)
+  while (R > budget && g < D && (n == 2 || 2 * (g + 1) <= D))
   {
     R /= n;
     g++;
