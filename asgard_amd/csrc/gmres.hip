@@ -331,8 +331,13 @@ static void delete_workspace(akm_gmres_workspace *w) { delete w; }
 
 namespace akm
 {
-//! Rows of a vector handled by one reduction block.
-static int64_t rows_per_block_for(int64_t n) { return std::max<int64_t>(4096, (n + 1023) / 1024); }
+//! Rows of a vector handled by one reduction block: at least 256 (one row per
+//! thread), otherwise enough blocks to cover the chip (about 1024) and no
+//! more, so the per-column partial sums stay short.
+static int64_t rows_per_block_for(int64_t n)
+{
+  return std::max<int64_t>(256, (n + 1023) / 1024);
+}
 
 //! One restart cycle's device state and the kernel sequence of its steps.
 template<typename P>
