@@ -44,7 +44,8 @@ AKM_F32 = 1
 EXPORTS = (
     "akm_last_error", "akm_version", "akm_device_count", "akm_set_stream",
     "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
-    "akm_create_sparse_grid", "akm_create_dense", "akm_update_coefficients",
+    "akm_create_sparse_grid", "akm_create_global_grid", "akm_global_info",
+    "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
     "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
@@ -108,6 +109,11 @@ def load_library(path: str = LIB_PATH):
                                     ctypes.POINTER(_vp)]
     L.akm_create_sparse_grid.argtypes = [_i, _i, _i64, _vp, _i, _i, _vp, _i64, _i,
                                          ctypes.POINTER(_vp)]
+    L.akm_create_global_grid.argtypes = [_i, _i, _i64, _vp, _i, _i, _vp, _i64, _vp, _vp, _i,
+                                         ctypes.POINTER(_vp)]
+    L.akm_global_info.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i),
+                                  ctypes.POINTER(_i)]
+    L.akm_global_jacobi.argtypes = [_vp, _vp]
     L.akm_create_dense.argtypes = [_i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i,
                                    ctypes.POINTER(_vp)]
     L.akm_update_coefficients.argtypes = [_vp, _vp]
@@ -264,6 +270,43 @@ class KronmultMatrix:
                                         int(max_level), num_terms, vA.ctypes.data, vA.size,
                                         prec, ctypes.byref(h)))
         return cls(h, prec, d, n, table.shape[0], table.shape[0], num_terms)
+
+    @classmethod
+    def global_grid(cls, d, n, table, max_level, num_terms, vA, dtype=np.float64, active=None,
+                    flux_dim=None):
+        """make_global_kron_matrix equivalent (the reference's global_kron_matrix)
+        from the same table and vA as sparse_grid; active[t][k] = 0 marks an
+        identity factor, flux_dim[t] the term's flux direction."""
+        L = load_library()
+        prec = AKM_F64 if np.dtype(dtype) == np.float64 else AKM_F32
+        table = _np(table, np.int32)
+        vA = _np(vA, dtype)
+        act = None if active is None else _np(np.asarray(active).reshape(-1), np.int32)
+        flux = None if flux_dim is None else _np(np.asarray(flux_dim).reshape(-1), np.int32)
+        if act is not None and act.size != num_terms * d:
+            raise ValueError("active needs num_terms x d entries")
+        if flux is not None and flux.size != num_terms:
+            raise ValueError("flux_dim needs num_terms entries")
+        h = _vp()
+        _check(L.akm_create_global_grid(d, n, table.shape[0], table.ctypes.data,
+                                        int(max_level), num_terms, vA.ctypes.data, vA.size,
+                                        None if act is None else act.ctypes.data,
+                                        None if flux is None else flux.ctypes.data,
+                                        prec, ctypes.byref(h)))
+        return cls(h, prec, d, n, table.shape[0], table.shape[0], num_terms)
+
+    def global_info(self):
+        """(padding cells, chains, stages) of a global operator."""
+        p, c, st = _i64(), _i(), _i()
+        _check(load_library().akm_global_info(self._h, ctypes.byref(p), ctypes.byref(c),
+                                              ctypes.byref(st)))
+        return p.value, c.value, st.value
+
+    def jacobi(self):
+        """The Jacobi diagonal of a global operator (FP64, active DOFs)."""
+        pc = np.zeros(self.output_size(), dtype=np.float64)
+        _check(load_library().akm_global_jacobi(self._h, pc.ctypes.data))
+        return pc
 
     @classmethod
     def dense(cls, d, n, rows, cols, num_terms, terms, elem, row_offset, col_offset,

@@ -1,0 +1,681 @@
+// Global Kronecker operator: the reference's default single-node kronmult
+// mode (global_kron_matrix, src/asgard_kronmult_matrix.hpp:780-989,
+// src/asgard_kronmult_matrix.cpp:1008-1590), on the GPU.
+//
+// The operator sum_t (A_t0 x ... x A_t,d-1) restricted to the grid's DOFs is
+// applied one dimension at a time.  Every term with k non-identity factors
+// becomes 2^(k-1) chains of k one-dimensional products over the cell set
+// completed with its missing ancestors ("padded"); in each chain the upper
+// parts (diagonal included) of some factors act first, one factor acts whole,
+// and the strictly lower parts of the others act last
+// (src/device/asgard_kronmult.hpp:152-220).  On a set closed under taking
+// ancestors in every dimension this reproduces the pair-by-pair operator
+// exactly for hierarchical (volume) couplings, and for every coupling on
+// standard sparse grids.
+//
+// GPU layout: DOF vectors keep the pair operator's layout (cell-major, n^d
+// tensor per cell, dimension 0 slowest), padded cells appended after the
+// active ones.  A one-dimensional product in dimension k is one thread per
+// (chain, DOF): the thread walks the connect_1d row of its 1D cell and finds
+// each connected cell of its fibre (cells equal in all other dimensions)
+// through a per-fibre table from 1D index to cell id.  The factor values are
+// the same connect_1d blocks (vA) the pair operator uses.  All chains of a
+// term set advance together, one launch per stage; the last stage of every
+// chain, the sum over chains and alpha/beta/gamma (and optionally the
+// reference's Jacobi scaling) are one launch.
+#include "akm_internal.hpp"
+#include "grid.hpp"
+#include "matrix.hpp"
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace akm
+{
+constexpr int gk_max_dims = 6;
+constexpr int gk_threads = 256;
+
+enum gk_fill : int
+{
+  gk_upper = 0, // connect_1d row from the diagonal on (diagonal included)
+  gk_full = 1,
+  gk_lower = 2, // strictly before the diagonal
+};
+
+//! Device view of the global operator.
+struct gk_args
+{
+  int d, n, N, T;
+  int64_t active_cells, cells; // active first, then padding
+  const int *elem;             // cells x d 1D indices
+  const int *c1_pntr, *c1_indx, *c1_diag; // connect_1d (full hierarchy)
+  const int *fib_of[gk_max_dims];   // cell -> fibre, per dimension
+  const int *fib_base[gk_max_dims]; // fibre -> offset into fib_map
+  const int *fib_len[gk_max_dims];  // fibre -> largest 1D index + 1
+  const int *fib_map[gk_max_dims];  // (fibre, 1D index) -> cell or -1
+  const int *ch_term;  // chain -> term
+  const int *ch_nst;   // chain -> number of stages
+  const int *ch_dim;   // chain -> dims of its stages (gk_max_dims each)
+  const int *ch_fill;  // chain -> fills of its stages
+  const int *ch_slot;  // chain -> work-buffer slot (-1 without intermediates)
+  int nchains;
+  int nslots;          // chains with intermediate stages
+  int64_t stride;      // work-buffer slot size (padded DOFs)
+};
+
+//! One stage's product for DOF i of a chain: (fill part of A_{t,k}) applied
+//! along dimension k to `in` (padded vector, or the active-only x when
+//! `in_active_only`).
+template<typename P>
+__device__ __forceinline__ P gk_row(gk_args const &a, const P *__restrict__ vA, int t, int k,
+                                    int fill, const P *__restrict__ in, bool in_active_only,
+                                    int64_t i)
+{
+  const int N = a.N, n = a.n, d = a.d;
+  const int64_t cell = i / N;
+  const int e = static_cast<int>(i - cell * N);
+  int stride_k = 1;
+  for (int j = k + 1; j < d; j++)
+    stride_k *= n;
+  const int s = (e / stride_k) % n;
+  const int e0 = e - s * stride_k;
+  const int r1 = a.elem[cell * d + k];
+  const int f = a.fib_of[k][cell];
+  const int *fmap = a.fib_map[k] + a.fib_base[k][f];
+  const int flen = a.fib_len[k][f];
+  const int qd = a.c1_diag[r1];
+  const int qbeg = (fill == gk_upper) ? qd : a.c1_pntr[r1];
+  const int qend = (fill == gk_lower) ? qd + 1 : a.c1_pntr[r1 + 1];
+  const int64_t nn = int64_t(n) * n;
+  const int64_t tstride = int64_t(d) * nn;
+  const int64_t cstride = tstride * a.T;
+  const P *Ab = vA + int64_t(t) * tstride + k * nn + s;
+  P acc = 0;
+  for (int q = qbeg; q < qend; q++)
+  {
+    const int c1 = a.c1_indx[q];
+    if (c1 >= flen)
+      break; // columns ascend; the fibre holds nothing beyond
+    const int cj = fmap[c1];
+    if (cj < 0 || (in_active_only && cj >= a.active_cells))
+      continue;
+    int jlo = 0, jhi = n;
+    if (q == qd)
+    {
+      if (fill == gk_upper)
+        jlo = s;
+      else if (fill == gk_lower)
+        jhi = s;
+    }
+    const P *A = Ab + int64_t(q) * cstride; // A[j*n] = A(s, j)
+    const P *xin = in + int64_t(cj) * N + e0;
+    for (int j = jlo; j < jhi; j++)
+      acc = fma(A[j * n], xin[j * stride_k], acc);
+  }
+  return acc;
+}
+
+//! Intermediate stage `st` of every chain with more than st + 1 stages.
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gk_stage(gk_args a, const P *__restrict__ vA, const int *__restrict__ list, int nlist, int st,
+             const P *__restrict__ x, P *work)
+{
+  const int64_t dofs = a.cells * a.N;
+  const int64_t total = dofs * nlist;
+  for (int64_t g = blockIdx.x * int64_t(gk_threads) + threadIdx.x; g < total;
+       g += int64_t(gk_threads) * gridDim.x)
+  {
+    const int li = static_cast<int>(g / dofs);
+    const int64_t i = g - int64_t(li) * dofs;
+    const int c = list[li];
+    const int slot = a.ch_slot[c];
+    const int k = a.ch_dim[c * gk_max_dims + st];
+    const int fill = a.ch_fill[c * gk_max_dims + st];
+    const P *in = (st == 0) ? x : work + (int64_t((st - 1) & 1) * a.nslots + slot) * a.stride;
+    P *out = work + (int64_t(st & 1) * a.nslots + slot) * a.stride;
+    out[i] = gk_row<P>(a, vA, a.ch_term[c], k, fill, in, st == 0, i);
+  }
+}
+
+//! Last stage of every chain, summed over the chains in chain order:
+//! y = (alpha * sum + beta*y + gamma*xg) / (1 - jdt * pc) (no division
+//! when pc is null).
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gk_final(gk_args a, const P *__restrict__ vA, const P *__restrict__ x, const P *work, P alpha,
+             P beta, P *y, P gamma, const P *xg, const double *pc, double jdt)
+{
+  const int64_t dofs = a.active_cells * a.N;
+  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < dofs;
+       i += int64_t(gk_threads) * gridDim.x)
+  {
+    P sum = 0;
+    for (int c = 0; c < a.nchains; c++)
+    {
+      const int ns = a.ch_nst[c];
+      if (ns == 0)
+      {
+        sum += x[i]; // a term of identity factors only
+        continue;
+      }
+      const int st = ns - 1;
+      const P *in = (st == 0) ? x
+                              : work + (int64_t((st - 1) & 1) * a.nslots + a.ch_slot[c]) *
+                                           a.stride;
+      sum += gk_row<P>(a, vA, a.ch_term[c], a.ch_dim[c * gk_max_dims + st],
+                       a.ch_fill[c * gk_max_dims + st], in, st == 0, i);
+    }
+    P out = alpha * sum;
+    if (beta != P{0})
+      out += beta * y[i];
+    if (gamma != P{0})
+      out += gamma * xg[i];
+    if (pc != nullptr)
+      out = static_cast<P>(static_cast<double>(out) / (1.0 - jdt * pc[i]));
+    y[i] = out;
+  }
+}
+
+//! y = beta*y + gamma*xg (no terms), with the optional Jacobi division.
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gk_scale(int64_t dofs, P beta, P *y, P gamma, const P *xg, const double *pc, double jdt)
+{
+  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < dofs;
+       i += int64_t(gk_threads) * gridDim.x)
+  {
+    P out = (beta != P{0}) ? beta * y[i] : P{0};
+    if (gamma != P{0})
+      out += gamma * xg[i];
+    if (pc != nullptr)
+      out = static_cast<P>(static_cast<double>(out) / (1.0 - jdt * pc[i]));
+    y[i] = out;
+  }
+}
+
+static int gk_blocks(int64_t work)
+{
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((work + gk_threads - 1) /
+                                                                     gk_threads,
+                                                                 65536)));
+}
+} // namespace akm
+
+//! Host and device data of a global operator.
+struct akm_global_s
+{
+  int64_t active_cells = 0, cells = 0;
+  std::vector<int> elem_host; // padded cells x d
+  akm::dev_buffer<int> elem, c1_pntr, c1_indx, c1_diag;
+  akm::dev_buffer<int> fib_of[akm::gk_max_dims], fib_base[akm::gk_max_dims],
+      fib_len[akm::gk_max_dims], fib_map[akm::gk_max_dims];
+  // chains
+  std::vector<int> ch_nst_host;
+  akm::dev_buffer<int> ch_term, ch_nst, ch_dim, ch_fill, ch_slot;
+  std::vector<akm::dev_buffer<int>> stage_list; // per intermediate stage: chains
+  std::vector<int> stage_count;
+  int nchains = 0, nslots = 0, max_stages = 0;
+  akm::dev_buffer<unsigned char> work;
+  akm::dev_buffer<double> pc; // Jacobi diagonal (active DOFs)
+  std::vector<int> c1_pntr_h, c1_indx_h, c1_diag_h;
+  int64_t flops = 0;
+};
+
+namespace akm
+{
+void destroy_global(akm_global_s *g) { delete g; }
+
+static gk_args global_args(akm_matrix_s const &m)
+{
+  const akm_global_s &g = *m.gk;
+  gk_args a{};
+  a.d = m.d;
+  a.n = m.n;
+  a.N = static_cast<int>(m.N);
+  a.T = m.T;
+  a.active_cells = g.active_cells;
+  a.cells = g.cells;
+  a.elem = g.elem.data();
+  a.c1_pntr = g.c1_pntr.data();
+  a.c1_indx = g.c1_indx.data();
+  a.c1_diag = g.c1_diag.data();
+  for (int k = 0; k < m.d; k++)
+  {
+    a.fib_of[k] = g.fib_of[k].data();
+    a.fib_base[k] = g.fib_base[k].data();
+    a.fib_len[k] = g.fib_len[k].data();
+    a.fib_map[k] = g.fib_map[k].data();
+  }
+  a.ch_term = g.ch_term.data();
+  a.ch_nst = g.ch_nst.data();
+  a.ch_dim = g.ch_dim.data();
+  a.ch_fill = g.ch_fill.data();
+  a.ch_slot = g.ch_slot.data();
+  a.nchains = g.nchains;
+  a.nslots = g.nslots;
+  a.stride = g.cells * m.N;
+  return a;
+}
+
+template<typename P>
+static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gamma,
+                              const P *xg, bool jacobi, double jdt)
+{
+  akm_global_s &g = *m.gk;
+  const int64_t dofs = g.active_cells * m.N;
+  const double *pc = jacobi ? g.pc.data() : nullptr;
+  if (g.nchains == 0 || alpha == P{0} || dofs == 0)
+  {
+    if (dofs > 0)
+      gk_scale<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(dofs, beta, y, gamma, xg, pc,
+                                                                  jdt);
+    return check_launch("global kron scale");
+  }
+  const gk_args a = global_args(m);
+  const P *vA = reinterpret_cast<const P *>(m.vA.data());
+  P *work = reinterpret_cast<P *>(g.work.data());
+  for (int st = 0; st + 1 < g.max_stages; st++)
+  {
+    const int nl = g.stage_count[st];
+    if (nl == 0)
+      continue;
+    gk_stage<P><<<gk_blocks(g.cells * m.N * nl), gk_threads, 0, m.stream>>>(
+        a, vA, g.stage_list[st].data(), nl, st, x, work);
+    AKM_TRY(check_launch("global kron stage"));
+  }
+  gk_final<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, vA, x, work, alpha, beta, y,
+                                                             gamma, xg, pc, jdt);
+  return check_launch("global kron final stage");
+}
+
+int global_apply(akm_matrix_s &m, double alpha, const void *x, double beta, void *y,
+                 double gamma, const void *xg, bool jacobi, double jdt)
+{
+  if (m.prec == AKM_F64)
+    return global_apply_typed<double>(m, alpha, static_cast<const double *>(x), beta,
+                                      static_cast<double *>(y), gamma,
+                                      static_cast<const double *>(xg), jacobi, jdt);
+  return global_apply_typed<float>(m, static_cast<float>(alpha), static_cast<const float *>(x),
+                                   static_cast<float>(beta), static_cast<float *>(y),
+                                   static_cast<float>(gamma), static_cast<const float *>(xg),
+                                   jacobi, jdt);
+}
+
+//! The Jacobi diagonal of the reference's implicit step
+//! (build_preconditioner, src/asgard_kronmult_matrix.cpp:1016-1067): for
+//! every active DOF, the sum over terms of the product over dimensions of the
+//! 1D factors' diagonal entries (every dimension, identity factors included).
+template<typename P>
+static void jacobi_diagonal(akm_matrix_s const &m, const P *vA, std::vector<double> &pc)
+{
+  const akm_global_s &g = *m.gk;
+  const int d = m.d, n = m.n, T = m.T;
+  const int64_t N = m.N, nn = int64_t(n) * n;
+  pc.assign(g.active_cells * N, 0.0);
+  for (int64_t c = 0; c < g.active_cells; c++)
+    for (int64_t e = 0; e < N; e++)
+    {
+      double sum = 0;
+      for (int t = 0; t < T; t++)
+      {
+        double prod = 1;
+        int64_t rest = e;
+        for (int k = d - 1; k >= 0; k--)
+        {
+          const int s = static_cast<int>(rest % n);
+          rest /= n;
+          const int r1 = g.elem_host[c * d + k];
+          const int64_t q = g.c1_diag_h[r1];
+          prod *= static_cast<double>(vA[((q * T + t) * d + k) * nn + int64_t(s) * n + s]);
+        }
+        sum += prod;
+      }
+      pc[c * N + e] = sum;
+    }
+}
+
+int global_update(akm_matrix_s &m, const void *vA)
+{
+  AKM_HIP(hipMemcpyAsync(m.vA.data(), vA, m.num_vA * m.esize, hipMemcpyHostToDevice, m.stream));
+  std::vector<double> pc;
+  if (m.prec == AKM_F64)
+    jacobi_diagonal(m, static_cast<const double *>(vA), pc);
+  else
+    jacobi_diagonal(m, static_cast<const float *>(vA), pc);
+  AKM_TRY(m.gk->pc.upload(pc.data(), int64_t(pc.size()), m.stream));
+  AKM_HIP(hipStreamSynchronize(m.stream));
+  return AKM_SUCCESS;
+}
+
+int64_t global_flops(akm_matrix_s const &m) { return m.gk ? m.gk->flops : 0; }
+
+//! Ancestors of a 1D cell in the volume hierarchy: its parent chain down to
+//! cell 0 (the part of a connect_1d volume row before the diagonal,
+//! src/asgard_grid_1d.hpp:159-260).
+static void parents_1d(int i, std::vector<int> &out)
+{
+  out.clear();
+  while (i > 0)
+  {
+    i = (i == 1) ? 0 : i / 2;
+    out.push_back(i);
+  }
+}
+
+//! Builds the padded cell set, the fibres, the chains and the work buffers
+//! (make_global_kron_matrix, src/asgard_kronmult_matrix.cpp:1198-1334).
+static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
+                        const int *active_mask, const int *flux_dim)
+{
+  auto g = std::unique_ptr<akm_global_s, void (*)(akm_global_s *)>(new akm_global_s(),
+                                                                    destroy_global);
+  const int d = m.d;
+  const int64_t nact = m.rows;
+  g->active_cells = nact;
+
+  // ancestry completion (compute_ancestry_completion over the volume
+  // hierarchy, src/asgard_indexset.cpp:243-318): add the missing ancestors
+  // in every dimension until none is missing; padding sorted
+  std::map<std::vector<int>, int64_t> index;
+  std::vector<int> key(d), anc;
+  for (int64_t c = 0; c < nact; c++)
+  {
+    key.assign(elem_active.begin() + c * d, elem_active.begin() + (c + 1) * d);
+    index.emplace(key, c);
+  }
+  std::vector<std::vector<int>> frontier, pad;
+  for (int64_t c = 0; c < nact; c++)
+    frontier.emplace_back(elem_active.begin() + c * d, elem_active.begin() + (c + 1) * d);
+  std::map<std::vector<int>, int> padset;
+  while (!frontier.empty())
+  {
+    std::vector<std::vector<int>> next;
+    for (auto const &cell : frontier)
+      for (int k = 0; k < d; k++)
+      {
+        parents_1d(cell[k], anc);
+        for (int a : anc)
+        {
+          key = cell;
+          key[k] = a;
+          if (index.count(key) == 0 && padset.count(key) == 0)
+          {
+            padset.emplace(key, 0);
+            next.push_back(key);
+          }
+        }
+      }
+    frontier.swap(next);
+  }
+  g->elem_host = elem_active;
+  for (auto const &kv : padset)
+  {
+    index.emplace(kv.first, nact + static_cast<int64_t>(g->elem_host.size() / d - nact));
+    g->elem_host.insert(g->elem_host.end(), kv.first.begin(), kv.first.end());
+  }
+  g->cells = static_cast<int64_t>(g->elem_host.size() / d);
+  if (g->cells * m.N > (int64_t(1) << 40))
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: too many padded DOFs");
+  AKM_TRY(g->elem.upload(g->elem_host.data(), int64_t(g->elem_host.size())));
+
+  // connect_1d (full hierarchy), the pattern of every 1D factor
+  AKM_TRY(build_connect_1d(m.max_level, true, g->c1_pntr_h, g->c1_indx_h, g->c1_diag_h));
+  AKM_TRY(g->c1_pntr.upload(g->c1_pntr_h.data(), int64_t(g->c1_pntr_h.size())));
+  AKM_TRY(g->c1_indx.upload(g->c1_indx_h.data(), int64_t(g->c1_indx_h.size())));
+  AKM_TRY(g->c1_diag.upload(g->c1_diag_h.data(), int64_t(g->c1_diag_h.size())));
+
+  // fibres per dimension: cells equal in every other 1D index; table from
+  // the 1D index in dimension k to the cell
+  std::vector<int64_t> nnz_dim(d, 0); // DOF-level pattern entries per dimension
+  for (int k = 0; k < d; k++)
+  {
+    std::map<std::vector<int>, int> fib;
+    std::vector<int> fib_of(g->cells), fib_len;
+    std::vector<std::vector<std::pair<int, int>>> members;
+    std::vector<int> other(d - 1);
+    for (int64_t c = 0; c < g->cells; c++)
+    {
+      for (int j = 0, o = 0; j < d; j++)
+        if (j != k)
+          other[o++] = g->elem_host[c * d + j];
+      auto it = fib.find(other);
+      if (it == fib.end())
+      {
+        it = fib.emplace(other, static_cast<int>(members.size())).first;
+        members.emplace_back();
+      }
+      fib_of[c] = it->second;
+      members[it->second].emplace_back(g->elem_host[c * d + k], static_cast<int>(c));
+    }
+    std::vector<int> base(members.size()), len(members.size()), map;
+    for (size_t f = 0; f < members.size(); f++)
+    {
+      int mx = 0;
+      for (auto const &p : members[f])
+        mx = std::max(mx, p.first + 1);
+      base[f] = static_cast<int>(map.size());
+      len[f] = mx;
+      if (map.size() + size_t(mx) > (size_t(1) << 31) - 1)
+        return set_error(AKM_ERR_UNSUPPORTED, "global kron: fibre tables too large");
+      map.resize(map.size() + mx, -1);
+      for (auto const &p : members[f])
+        map[base[f] + p.first] = p.second;
+      for (auto const &p : members[f])
+        for (int q = g->c1_pntr_h[p.first]; q < g->c1_pntr_h[p.first + 1]; q++)
+        {
+          const int c1 = g->c1_indx_h[q];
+          if (c1 < mx && map[base[f] + c1] >= 0)
+            nnz_dim[k] += int64_t(m.n) * m.N;
+        }
+    }
+    AKM_TRY(g->fib_of[k].upload(fib_of.data(), int64_t(fib_of.size())));
+    AKM_TRY(g->fib_base[k].upload(base.data(), int64_t(base.size())));
+    AKM_TRY(g->fib_len[k].upload(len.data(), int64_t(len.size())));
+    AKM_TRY(g->fib_map[k].upload(map.data(), int64_t(map.size())));
+  }
+
+  // chains (kronmult::permutes, src/device/asgard_kronmult.hpp:152-220, and
+  // the active-direction order of src/asgard_kronmult_matrix.cpp:1313-1331)
+  std::vector<int> ch_term, ch_nst, ch_dim, ch_fill, ch_slot;
+  for (int t = 0; t < m.T; t++)
+  {
+    std::vector<int> act;
+    for (int k = 0; k < d; k++)
+      if (active_mask == nullptr || active_mask[t * d + k] != 0)
+        act.push_back(k);
+    const int na = static_cast<int>(act.size());
+    if (na > 1 && flux_dim != nullptr)
+    {
+      // the reference swaps positions 0 and flux_dim of the active list
+      const int fd = flux_dim[t];
+      if (fd >= 0 && fd < na && fd != act[0])
+        std::swap(act[0], act[fd]);
+    }
+    if (na == 0)
+    {
+      ch_term.push_back(t);
+      ch_nst.push_back(0);
+      ch_dim.insert(ch_dim.end(), gk_max_dims, 0);
+      ch_fill.insert(ch_fill.end(), gk_max_dims, 0);
+      continue;
+    }
+    for (int p = 0; p < (1 << (na - 1)); p++)
+    {
+      std::vector<int> sgn(1, 0);
+      int bits = p;
+      for (int dd = 1; dd < na; dd++)
+      {
+        sgn.push_back((bits % 2 == 0) ? dd : -dd);
+        bits /= 2;
+      }
+      std::sort(sgn.begin(), sgn.end());
+      ch_term.push_back(t);
+      ch_nst.push_back(na);
+      for (int st = 0; st < gk_max_dims; st++)
+      {
+        if (st < na)
+        {
+          ch_dim.push_back(act[std::abs(sgn[st])]);
+          ch_fill.push_back(sgn[st] < 0 ? gk_upper : (sgn[st] > 0 ? gk_lower : gk_full));
+        }
+        else
+        {
+          ch_dim.push_back(0);
+          ch_fill.push_back(0);
+        }
+      }
+    }
+  }
+  g->nchains = static_cast<int>(ch_term.size());
+  g->max_stages = 0;
+  int nslots = 0;
+  for (int c = 0; c < g->nchains; c++)
+  {
+    g->max_stages = std::max(g->max_stages, ch_nst[c]);
+    ch_slot.push_back(ch_nst[c] >= 2 ? nslots++ : -1);
+  }
+  g->nslots = nslots;
+  g->ch_nst_host = ch_nst;
+  if (g->nchains > 0)
+  {
+    AKM_TRY(g->ch_term.upload(ch_term.data(), int64_t(ch_term.size())));
+    AKM_TRY(g->ch_nst.upload(ch_nst.data(), int64_t(ch_nst.size())));
+    AKM_TRY(g->ch_dim.upload(ch_dim.data(), int64_t(ch_dim.size())));
+    AKM_TRY(g->ch_fill.upload(ch_fill.data(), int64_t(ch_fill.size())));
+    AKM_TRY(g->ch_slot.upload(ch_slot.data(), int64_t(ch_slot.size())));
+  }
+  // chains per intermediate stage (those with a later stage)
+  g->stage_list.resize(std::max(0, g->max_stages - 1));
+  g->stage_count.assign(std::max(0, g->max_stages - 1), 0);
+  for (int st = 0; st + 1 < g->max_stages; st++)
+  {
+    std::vector<int> list;
+    for (int c = 0; c < g->nchains; c++)
+      if (ch_nst[c] >= st + 2)
+        list.push_back(c);
+    g->stage_count[st] = static_cast<int>(list.size());
+    if (!list.empty())
+      AKM_TRY(g->stage_list[st].upload(list.data(), int64_t(list.size())));
+  }
+  if (nslots > 0)
+    AKM_TRY(g->work.allocate(int64_t(2) * nslots * g->cells * m.N * int64_t(m.esize)));
+  // flops as the reference counts them (src/asgard_kronmult_matrix.cpp:
+  // 1389-1395): two per entry of the full DOF-level pattern of every
+  // non-identity (term, dimension) on the padded set
+  for (int t = 0; t < m.T; t++)
+    for (int k = 0; k < d; k++)
+      if (active_mask == nullptr || active_mask[t * d + k] != 0)
+        g->flops += 2 * nnz_dim[k];
+  m.gk = std::move(g);
+  return AKM_SUCCESS;
+}
+} // namespace akm
+
+using namespace akm;
+
+extern "C" {
+
+akm_status akm_create_global_grid(int d, int n, int64_t num_cells, const int *table,
+                                  int max_level, int T, const void *vA, int64_t num_vA,
+                                  const int *active, const int *flux_dim, akm_prec prec,
+                                  akm_matrix *mat)
+{
+  if (mat == nullptr)
+    return set_error(AKM_ERR_INVALID, "null matrix handle pointer");
+  *mat = nullptr;
+  if (d < 1 || d > gk_max_dims)
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: d = %d not in [1, 6]", d);
+  if (n < 1 || n > 16)
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: n = %d not in [1, 16]", n);
+  if (num_cells < 0 || num_cells > (int64_t(1) << 31) - 2 || T < 0)
+    return set_error(AKM_ERR_INVALID, "global kron: invalid sizes");
+  if (prec != AKM_F64 && prec != AKM_F32)
+    return set_error(AKM_ERR_INVALID, "global kron: unknown precision %d", int(prec));
+  if (table == nullptr && num_cells > 0)
+    return set_error(AKM_ERR_INVALID, "global kron: null table");
+  if (max_level < 0 || max_level > 20)
+    return set_error(AKM_ERR_INVALID, "global kron: max_level %d not in [0, 20]", max_level);
+  for (int64_t c = 0; c < num_cells; c++)
+    for (int k = 0; k < d; k++)
+    {
+      const int L = table[c * 2 * d + k], p = table[c * 2 * d + d + k];
+      if (L < 0 || L > max_level || p < 0 || (L <= 1 && p != 0) ||
+          (L > 1 && p >= (1 << (L - 1))))
+        return set_error(AKM_ERR_INVALID, "global kron: invalid cell (%d, %d)", L, p);
+    }
+  std::vector<int> p1, i1, dg;
+  AKM_TRY(build_connect_1d(max_level, true, p1, i1, dg));
+  const int64_t nconn = static_cast<int64_t>(i1.size());
+  const int64_t expect_vA = nconn * d * T * int64_t(n) * n;
+  if (T > 0 && (vA == nullptr || num_vA != expect_vA))
+    return set_error(AKM_ERR_INVALID,
+                     "global kron: vA has %lld values, connect_1d(%d) needs %lld",
+                     static_cast<long long>(num_vA), max_level,
+                     static_cast<long long>(expect_vA));
+  std::vector<int> elem;
+  cell_1d_indices(d, num_cells, table, elem);
+  {
+    std::map<std::vector<int>, int> seen;
+    for (int64_t c = 0; c < num_cells; c++)
+      if (!seen.emplace(std::vector<int>(elem.begin() + c * d, elem.begin() + (c + 1) * d), 0)
+               .second)
+        return set_error(AKM_ERR_INVALID, "global kron: cell %lld repeated",
+                         static_cast<long long>(c));
+  }
+  std::unique_ptr<akm_matrix_s> m(new akm_matrix_s());
+  m->prec = prec;
+  m->esize = (prec == AKM_F64) ? 8 : 4;
+  m->d = d;
+  m->n = n;
+  m->rows = m->cols = static_cast<int>(num_cells);
+  m->T = T;
+  int64_t N = 1;
+  for (int k = 0; k < d; k++)
+    N *= n;
+  if (N > 4096)
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: n^d = %lld exceeds 4096",
+                     static_cast<long long>(N));
+  m->N = N;
+  m->mode = mode_global;
+  m->max_level = max_level;
+  m->M = 1 << max_level;
+  m->num_vA = (T > 0) ? num_vA : 0;
+  m->nconn = static_cast<int>(nconn);
+  if (T > 0)
+    AKM_TRY(m->vA.upload(static_cast<const unsigned char *>(vA), num_vA * m->esize));
+  AKM_TRY(build_global(*m, elem, active, flux_dim));
+  if (T > 0)
+    AKM_TRY(global_update(*m, vA));
+  *mat = m.release();
+  return AKM_SUCCESS;
+}
+
+akm_status akm_global_info(akm_matrix mat, int64_t *padded_cells, int *chains, int *stages)
+{
+  if (mat == nullptr || mat->mode != mode_global || !mat->gk)
+    return set_error(AKM_ERR_INVALID, "global info: not a global kron matrix");
+  if (padded_cells)
+    *padded_cells = mat->gk->cells - mat->gk->active_cells;
+  if (chains)
+    *chains = mat->gk->nchains;
+  if (stages)
+    *stages = mat->gk->max_stages;
+  return AKM_SUCCESS;
+}
+
+akm_status akm_global_jacobi(akm_matrix mat, double *pc)
+{
+  if (mat == nullptr || mat->mode != mode_global || !mat->gk || pc == nullptr)
+    return set_error(AKM_ERR_INVALID, "global jacobi: not a global kron matrix");
+  const int64_t count = mat->gk->active_cells * mat->N;
+  if (count > 0 && mat->gk->pc.data() != nullptr)
+    AKM_HIP(hipMemcpy(pc, mat->gk->pc.data(), sizeof(double) * count, hipMemcpyDeviceToHost));
+  else if (count > 0)
+    std::fill(pc, pc + count, 0.0);
+  return AKM_SUCCESS;
+}
+
+} // extern "C"

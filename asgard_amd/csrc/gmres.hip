@@ -472,8 +472,16 @@ static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int
   hipStream_t s = m.stream;
   gm_cycle<P> cyc(*m.gmres, n, s, restart, nullptr, 0);
   P *V = cyc.V;
+  // the global operator's solve is left-preconditioned by its Jacobi
+  // diagonal, fused into the apply (src/solver.cpp:94-118)
+  const bool jacobi = (m.mode == mode_global);
+  auto apply_op = [&](P alpha, const P *v, P beta, P *w, P gamma, const P *g) -> int {
+    if (jacobi)
+      return global_apply(m, alpha, v, beta, w, gamma, g, true, static_cast<double>(dt));
+    return apply_device(m, alpha, v, beta, w, gamma, g);
+  };
   const std::function<int(const P *, P *)> op = [&](const P *v, P *w) -> int {
-    return apply_device(m, -dt, v, P{0}, w, P{1}, v);
+    return apply_op(-dt, v, P{0}, w, P{1}, v);
   };
 
   int total = 0, outer = 0;
@@ -482,7 +490,7 @@ static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int
   {
     // r = b - (I - dt A) x into V_0
     AKM_HIP(hipMemcpyAsync(V, b, sizeof(P) * n, hipMemcpyDeviceToDevice, s));
-    AKM_TRY(apply_device(m, -dt * P{-1}, x, P{1}, V, P{-1}, x));
+    AKM_TRY(apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x));
     ++total;
     AKM_TRY(cyc.start(tol, inner_res));
     int inner = 0;

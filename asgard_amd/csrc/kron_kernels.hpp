@@ -47,6 +47,8 @@ enum addr_mode : int
   mode_grid = 1,
   //! dense mode: all column cells, per-term block arrays (make_kronmult_dense)
   mode_dense = 2,
+  //! global Kronecker operator (global_kron_matrix), one dimension at a time
+  mode_global = 3,
 };
 
 template<typename P>

@@ -146,6 +146,18 @@ static kron_args<P> matrix_args(akm_matrix_s const &m)
 
 kron_args<double> kron_args_f64(akm_matrix_s const &m) { return matrix_args<double>(m); }
 
+} // namespace akm
+
+int64_t akm_matrix_s::flops() const
+{
+  if (mode == akm::mode_global)
+    return akm::global_flops(*this);
+  return N * n * d * int64_t(T) * nnz * 2;
+}
+
+namespace akm
+{
+
 template<typename P>
 static int apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gamma,
                        const P *xg)
@@ -168,6 +180,8 @@ int apply_device(akm_matrix_s &m, double alpha, const void *x, double beta, void
 {
   if (gamma != 0 && m.rows != m.cols)
     return set_error(AKM_ERR_INVALID, "identity term needs a square operator");
+  if (m.mode == mode_global)
+    return global_apply(m, alpha, x, beta, y, gamma, xg, false, 0.0);
   if (m.prec == AKM_F64)
     return apply_typed<double>(m, alpha, static_cast<const double *>(x), beta,
                                static_cast<double *>(y), gamma,
@@ -455,6 +469,11 @@ akm_status akm_update_coefficients(akm_matrix mat, const void *values)
                              hipMemcpyHostToDevice, mat->stream));
     }
   }
+  else if (mat->mode == mode_global)
+  {
+    if (mat->num_vA > 0)
+      AKM_TRY(global_update(*mat, values));
+  }
   else if (mat->num_vA > 0)
   {
     AKM_HIP(hipMemcpyAsync(mat->vA.data(), values, mat->num_vA * mat->esize,
@@ -556,6 +575,15 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
 {
   if (mat == nullptr || x == nullptr || y == nullptr)
     return set_error(AKM_ERR_INVALID, "autotune: null argument");
+  if (mat->mode == mode_global)
+  {
+    // one kernel shape: nothing to choose
+    if (waves_per_row)
+      *waves_per_row = 0;
+    if (lds_tables)
+      *lds_tables = 0;
+    return AKM_SUCCESS;
+  }
   struct choice
   {
     int waves, lds;

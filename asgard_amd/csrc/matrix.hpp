@@ -8,6 +8,7 @@
 #include "kron_kernels.hpp"
 
 struct akm_gmres_workspace;
+struct akm_global_s;
 
 struct akm_matrix_s
 {
@@ -54,10 +55,12 @@ struct akm_matrix_s
 
   std::unique_ptr<akm_gmres_workspace, void (*)(akm_gmres_workspace *)> gmres{nullptr,
                                                                              nullptr};
+  // global Kronecker operator (mode_global)
+  std::unique_ptr<akm_global_s, void (*)(akm_global_s *)> gk{nullptr, nullptr};
 
   int64_t input_size() const { return N * cols; }
   int64_t output_size() const { return N * rows; }
-  int64_t flops() const { return N * n * d * int64_t(T) * nnz * 2; }
+  int64_t flops() const;
 };
 
 namespace akm
@@ -68,6 +71,16 @@ int apply_device(akm_matrix_s &m, double alpha, const void *x, double beta, void
 
 //! Wavefronts per row workgroup used for the matrix (explicit or automatic).
 int waves_per_row(akm_matrix_s const &m);
+
+//! Global operator: y = alpha*A*x + beta*y + gamma*xg on device pointers,
+//! divided by (1 - jdt * pc) with the reference's Jacobi diagonal when
+//! `jacobi` is set.
+int global_apply(akm_matrix_s &m, double alpha, const void *x, double beta, void *y,
+                 double gamma, const void *xg, bool jacobi, double jdt);
+//! Global operator: new factor values (same vA layout), Jacobi diagonal rebuilt.
+int global_update(akm_matrix_s &m, const void *vA);
+//! Global operator: flops per apply as the reference counts them.
+int64_t global_flops(akm_matrix_s const &m);
 
 //! Kernel arguments of an FP64 matrix (vectors and scalars left empty).
 kron_args<double> kron_args_f64(akm_matrix_s const &m);

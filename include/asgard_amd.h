@@ -108,6 +108,37 @@ akm_status akm_create_sparse_grid(int d, int n, int64_t num_cells, const int *ta
                                   int max_level, int num_terms, const void *vA,
                                   int64_t num_vA, akm_prec prec, akm_matrix *mat);
 
+/* Global Kronecker factory: make_global_kron_matrix + set_specific_mode
+ * (src/asgard_kronmult_matrix.cpp:1198-1426), the reference's default
+ * single-node operator (global_kron_matrix<P>, src/asgard_kronmult_matrix.hpp:780-989).
+ * Same table and vA as akm_create_sparse_grid.  active (num_terms x d host
+ * ints, NULL = all) marks the non-identity factors: a factor marked 0 is
+ * taken as the identity, as the reference does for pure mass terms
+ * (check_identity_term, :1175-1184).  flux_dim (num_terms host ints, NULL =
+ * none) is the reference's per-term flux direction, which it moves to the
+ * front of the active list (:1313-1331).  The cell set is completed with its
+ * missing ancestors (compute_ancestry_completion); the operator is applied
+ * one dimension at a time (global_cpu, src/device/asgard_glkronmult_cpu.cpp:151-189).
+ * The handle works with akm_apply, akm_apply_euler, akm_update_coefficients
+ * (which also rebuilds the Jacobi diagonal), akm_time_apply, akm_matrix_info
+ * (flops counted as the reference does, num_pairs = 0) and the solvers; its
+ * akm_gmres_euler is the reference's left-Jacobi-preconditioned solve
+ * (simple_gmres_euler for global_kron_matrix, src/solver.cpp:94-118). */
+akm_status akm_create_global_grid(int d, int n, int64_t num_cells, const int *table,
+                                  int max_level, int num_terms, const void *vA,
+                                  int64_t num_vA, const int *active, const int *flux_dim,
+                                  akm_prec prec, akm_matrix *mat);
+
+/* Global operator sizes: padding cells added to the grid, chains (one per
+ * upper/lower choice of each term), stages per apply.  No counterpart in the
+ * reference (its constructor prints the padding count). */
+akm_status akm_global_info(akm_matrix mat, int64_t *padded_cells, int *chains, int *stages);
+
+/* The Jacobi diagonal of a global operator (build_preconditioner,
+ * src/asgard_kronmult_matrix.cpp:1016-1067), one FP64 value per active DOF,
+ * copied to the host array pc. */
+akm_status akm_global_jacobi(akm_matrix mat, double *pc);
+
 /* Dense constructor of kronmult_matrix<P>
  * (src/asgard_kronmult_matrix.hpp:132-176): every row/column pair, term t's
  * factors in terms[t] (d*nb*nb*n*n values each, block (ocol, orow) of dim k at
@@ -176,7 +207,11 @@ akm_status akm_destroy(akm_matrix mat);
  * restart/max_iter/tol <= 0 select the reference defaults
  * (default_gmres_restarts, n outer iterations, 1e-12 FP64 / 1e-6 FP32).
  * All vector work stays on the GPU.  err and iterations mirror gmres_info.
- * Returns AKM_ERR_INVALID if restart > n, as the reference throws. */
+ * Returns AKM_ERR_INVALID if restart > n, as the reference throws.  On a
+ * global operator (akm_create_global_grid) the residual and every new basis
+ * vector are divided by (1 - dt*pc), pc its Jacobi diagonal, as the
+ * reference's global-mode solve does (src/solver.cpp:76-118); tol and err
+ * then refer to that preconditioned residual. */
 akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, int restart,
                            int max_iter, double tol, int on_device, double *err,
                            int *iterations);

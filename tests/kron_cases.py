@@ -77,3 +77,47 @@ def bruteforce(case, alpha, beta):
                 acc += K @ xc
         y[r * N:(r + 1) * N] += alpha * acc
     return y
+
+
+def grid_case(d, n, L, T, seed=0, refine=0.0, coupling="full", identity=()):
+    """A sparse-grid case for the global operator: the |l|_1 <= L grid, or
+    (refine > 0) the |l|_1 <= L-2 grid refined twice by asgard_amd.grid.refine
+    up to level L; U[-1, 1] 1D operators, kept only on the volume connections
+    when coupling == "volume"; the (t, k) pairs in ``identity`` get the
+    identity operator and active[t][k] = 0."""
+    from asgard_amd import grid
+
+    rng = np.random.default_rng(seed)
+    if refine > 0:
+        table, _ = oracle.element_table([max(1, L - 2)] * d)
+        table = grid.refine(table, refine, seed=seed, max_level=L)
+        table = grid.refine(table, refine, seed=seed + 1, max_level=L)
+    else:
+        table, _ = oracle.element_table([L] * d)
+    size1d = n << L
+    mask = None
+    if coupling == "volume":
+        vp, vi, _ = oracle.connect_1d(L, full=False)
+        M = 1 << L
+        vm = np.zeros((M, M), bool)
+        for r in range(M):
+            vm[r, vi[vp[r]:vp[r + 1]]] = True
+        mask = np.kron(vm, np.ones((n, n), bool))
+    ops = []
+    for t in range(T):
+        for k in range(d):
+            if (t, k) in identity:
+                ops.append(np.asfortranarray(np.eye(size1d)))
+                continue
+            a = rng.uniform(-1, 1, size=(size1d, size1d))
+            if mask is not None:
+                a = np.where(mask, a, 0.0)
+            ops.append(np.asfortranarray(a))
+    vA = oracle.build_vA(d, n, T, L, ops)
+    pntr, indx, iA = oracle.sparse_pattern(table, n=n, T=T, max_level=L)
+    cells = table.shape[0]
+    N = n ** d
+    active = [[0 if (t, k) in identity else 1 for k in range(d)] for t in range(T)]
+    return dict(d=d, n=n, T=T, L=L, table=table, pntr=pntr, indx=indx, iA=iA, vA=vA,
+                x=rng.uniform(-1, 1, cells * N), y=rng.uniform(-1, 1, cells * N),
+                rows=cells, cols=cells, active=active)
