@@ -15,11 +15,14 @@
 //
 // GPU layout: DOF vectors keep the pair operator's layout (cell-major, n^d
 // tensor per cell, dimension 0 slowest), padded cells appended after the
-// active ones.  A one-dimensional product in dimension k is one thread per
-// (chain, DOF): the thread walks the connect_1d row of its 1D cell and finds
-// each connected cell of its fibre (cells equal in all other dimensions)
-// through a per-fibre table from 1D index to cell id.  The factor values are
-// the same connect_1d blocks (vA) the pair operator uses.  All chains of a
+// active ones.  For every dimension k and cell the host lists the connected
+// cells of its fibre (cells equal in all other 1D indices) with their
+// connect_1d positions.  A one-dimensional product in dimension k is one
+// lane per (chain, DOF), the lanes of a cell reading each connected cell's
+// tensor together, or (for small operators, where the longest lists set the
+// time) a group of 8 lanes per DOF that split the list and sum by a fixed
+// shuffle tree; akm_autotune times both.  The factor values are the same
+// connect_1d blocks (vA) the pair operator uses.  All chains of a
 // term set advance together, one launch per stage; the last stage of every
 // chain, the sum over chains and alpha/beta/gamma (and optionally the
 // reference's Jacobi scaling) are one launch.
@@ -39,7 +42,7 @@ constexpr int gk_threads = 256;
 
 enum gk_fill : int
 {
-  gk_upper = 0, // connect_1d row from the diagonal on (diagonal included)
+  gk_upper = 0, // from the diagonal on (diagonal included)
   gk_full = 1,
   gk_lower = 2, // strictly before the diagonal
 };
@@ -50,11 +53,12 @@ struct gk_args
   int d, n, N, T;
   int64_t active_cells, cells; // active first, then padding
   const int *elem;             // cells x d 1D indices
-  const int *c1_pntr, *c1_indx, *c1_diag; // connect_1d (full hierarchy)
-  const int *fib_of[gk_max_dims];   // cell -> fibre, per dimension
-  const int *fib_base[gk_max_dims]; // fibre -> offset into fib_map
-  const int *fib_len[gk_max_dims];  // fibre -> largest 1D index + 1
-  const int *fib_map[gk_max_dims];  // (fibre, 1D index) -> cell or -1
+  // per dimension: every cell's connected cells of its fibre (cells equal in
+  // all other 1D indices), 1D index ascending, with their connect_1d position
+  const int *nb_ptr[gk_max_dims];  // cells + 1
+  const int *nb_cell[gk_max_dims];
+  const int *nb_q[gk_max_dims];
+  const int *nb_diag[gk_max_dims]; // position of the cell itself
   const int *ch_term;  // chain -> term
   const int *ch_nst;   // chain -> number of stages
   const int *ch_dim;   // chain -> dims of its stages (gk_max_dims each)
@@ -65,13 +69,15 @@ struct gk_args
   int64_t stride;      // work-buffer slot size (padded DOFs)
 };
 
-//! One stage's product for DOF i of a chain: (fill part of A_{t,k}) applied
-//! along dimension k to `in` (padded vector, or the active-only x when
-//! `in_active_only`).
-template<typename P>
-__device__ __forceinline__ P gk_row(gk_args const &a, const P *__restrict__ vA, int t, int k,
-                                    int fill, const P *__restrict__ in, bool in_active_only,
-                                    int64_t i)
+//! Lane g's share (connections g, g + LANES, ...) of one stage's product for
+//! DOF i of a chain: (fill part of A_{t,k}) applied along dimension k to
+//! `in` (padded vector, or the active-only x when `in_active_only`).  With
+//! one lane per DOF the lanes of a cell read each connected cell's tensor
+//! together (coalesced); with several, long connection lists are split.
+template<typename P, int LANES>
+__device__ __forceinline__ P gk_row_part(gk_args const &a, const P *__restrict__ vA, int t,
+                                         int k, int fill, const P *__restrict__ in,
+                                         bool in_active_only, int64_t i, int g)
 {
   const int N = a.N, n = a.n, d = a.d;
   const int64_t cell = i / N;
@@ -81,35 +87,31 @@ __device__ __forceinline__ P gk_row(gk_args const &a, const P *__restrict__ vA, 
     stride_k *= n;
   const int s = (e / stride_k) % n;
   const int e0 = e - s * stride_k;
-  const int r1 = a.elem[cell * d + k];
-  const int f = a.fib_of[k][cell];
-  const int *fmap = a.fib_map[k] + a.fib_base[k][f];
-  const int flen = a.fib_len[k][f];
-  const int qd = a.c1_diag[r1];
-  const int qbeg = (fill == gk_upper) ? qd : a.c1_pntr[r1];
-  const int qend = (fill == gk_lower) ? qd + 1 : a.c1_pntr[r1 + 1];
+  const int dg = a.nb_diag[k][cell];
+  const int lo = (fill == gk_upper) ? dg : a.nb_ptr[k][cell];
+  const int hi = (fill == gk_lower) ? dg + 1 : a.nb_ptr[k][cell + 1];
   const int64_t nn = int64_t(n) * n;
   const int64_t tstride = int64_t(d) * nn;
   const int64_t cstride = tstride * a.T;
   const P *Ab = vA + int64_t(t) * tstride + k * nn + s;
+  const int *__restrict__ ncell = a.nb_cell[k];
+  const int *__restrict__ nq = a.nb_q[k];
   P acc = 0;
-  for (int q = qbeg; q < qend; q++)
+#pragma unroll 4
+  for (int p = lo + g; p < hi; p += LANES)
   {
-    const int c1 = a.c1_indx[q];
-    if (c1 >= flen)
-      break; // columns ascend; the fibre holds nothing beyond
-    const int cj = fmap[c1];
-    if (cj < 0 || (in_active_only && cj >= a.active_cells))
+    const int cj = ncell[p];
+    if (in_active_only && cj >= a.active_cells)
       continue;
     int jlo = 0, jhi = n;
-    if (q == qd)
+    if (p == dg)
     {
       if (fill == gk_upper)
         jlo = s;
       else if (fill == gk_lower)
         jhi = s;
     }
-    const P *A = Ab + int64_t(q) * cstride; // A[j*n] = A(s, j)
+    const P *A = Ab + int64_t(nq[p]) * cstride; // A[j*n] = A(s, j)
     const P *xin = in + int64_t(cj) * N + e0;
     for (int j = jlo; j < jhi; j++)
       acc = fma(A[j * n], xin[j * stride_k], acc);
@@ -117,40 +119,55 @@ __device__ __forceinline__ P gk_row(gk_args const &a, const P *__restrict__ vA, 
   return acc;
 }
 
-//! Intermediate stage `st` of every chain with more than st + 1 stages.
-template<typename P>
+//! Sum over the LANES lanes of a group, the same fixed tree on every run.
+template<typename P, int LANES>
+__device__ __forceinline__ P gk_group_sum(P v)
+{
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, LANES);
+  return v;
+}
+
+//! Intermediate stage `st` of every chain with more than st + 1 stages; one
+//! group of LANES lanes per (chain, padded DOF).
+template<typename P, int LANES>
 __global__ void __launch_bounds__(gk_threads)
     gk_stage(gk_args a, const P *__restrict__ vA, const int *__restrict__ list, int nlist, int st,
              const P *__restrict__ x, P *work)
 {
   const int64_t dofs = a.cells * a.N;
-  const int64_t total = dofs * nlist;
-  for (int64_t g = blockIdx.x * int64_t(gk_threads) + threadIdx.x; g < total;
-       g += int64_t(gk_threads) * gridDim.x)
+  const int64_t groups = dofs * nlist;
+  const int g = threadIdx.x % LANES;
+  for (int64_t gi = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / LANES; gi < groups;
+       gi += int64_t(gk_threads / LANES) * gridDim.x)
   {
-    const int li = static_cast<int>(g / dofs);
-    const int64_t i = g - int64_t(li) * dofs;
+    const int li = static_cast<int>(gi / dofs);
+    const int64_t i = gi - int64_t(li) * dofs;
     const int c = list[li];
     const int slot = a.ch_slot[c];
     const int k = a.ch_dim[c * gk_max_dims + st];
     const int fill = a.ch_fill[c * gk_max_dims + st];
     const P *in = (st == 0) ? x : work + (int64_t((st - 1) & 1) * a.nslots + slot) * a.stride;
-    P *out = work + (int64_t(st & 1) * a.nslots + slot) * a.stride;
-    out[i] = gk_row<P>(a, vA, a.ch_term[c], k, fill, in, st == 0, i);
+    const P v = gk_group_sum<P, LANES>(
+        gk_row_part<P, LANES>(a, vA, a.ch_term[c], k, fill, in, st == 0, i, g));
+    if (g == 0)
+      work[(int64_t(st & 1) * a.nslots + slot) * a.stride + i] = v;
   }
 }
 
-//! Last stage of every chain, summed over the chains in chain order:
-//! y = (alpha * sum + beta*y + gamma*xg) / (1 - jdt * pc) (no division
-//! when pc is null).
-template<typename P>
+//! Last stage of every chain, summed over the chains (each lane in chain
+//! order, then the group tree): y = (alpha * sum + beta*y + gamma*xg) /
+//! (1 - jdt * pc) (no division when pc is null).
+template<typename P, int LANES>
 __global__ void __launch_bounds__(gk_threads)
     gk_final(gk_args a, const P *__restrict__ vA, const P *__restrict__ x, const P *work, P alpha,
              P beta, P *y, P gamma, const P *xg, const double *pc, double jdt)
 {
   const int64_t dofs = a.active_cells * a.N;
-  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < dofs;
-       i += int64_t(gk_threads) * gridDim.x)
+  const int g = threadIdx.x % LANES;
+  for (int64_t i = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / LANES; i < dofs;
+       i += int64_t(gk_threads / LANES) * gridDim.x)
   {
     P sum = 0;
     for (int c = 0; c < a.nchains; c++)
@@ -158,24 +175,29 @@ __global__ void __launch_bounds__(gk_threads)
       const int ns = a.ch_nst[c];
       if (ns == 0)
       {
-        sum += x[i]; // a term of identity factors only
+        if (g == 0)
+          sum += x[i]; // a term of identity factors only
         continue;
       }
       const int st = ns - 1;
       const P *in = (st == 0) ? x
                               : work + (int64_t((st - 1) & 1) * a.nslots + a.ch_slot[c]) *
                                            a.stride;
-      sum += gk_row<P>(a, vA, a.ch_term[c], a.ch_dim[c * gk_max_dims + st],
-                       a.ch_fill[c * gk_max_dims + st], in, st == 0, i);
+      sum += gk_row_part<P, LANES>(a, vA, a.ch_term[c], a.ch_dim[c * gk_max_dims + st],
+                                   a.ch_fill[c * gk_max_dims + st], in, st == 0, i, g);
     }
-    P out = alpha * sum;
-    if (beta != P{0})
-      out += beta * y[i];
-    if (gamma != P{0})
-      out += gamma * xg[i];
-    if (pc != nullptr)
-      out = static_cast<P>(static_cast<double>(out) / (1.0 - jdt * pc[i]));
-    y[i] = out;
+    sum = gk_group_sum<P, LANES>(sum);
+    if (g == 0)
+    {
+      P out = alpha * sum;
+      if (beta != P{0})
+        out += beta * y[i];
+      if (gamma != P{0})
+        out += gamma * xg[i];
+      if (pc != nullptr)
+        out = static_cast<P>(static_cast<double>(out) / (1.0 - jdt * pc[i]));
+      y[i] = out;
+    }
   }
 }
 
@@ -209,15 +231,16 @@ struct akm_global_s
 {
   int64_t active_cells = 0, cells = 0;
   std::vector<int> elem_host; // padded cells x d
-  akm::dev_buffer<int> elem, c1_pntr, c1_indx, c1_diag;
-  akm::dev_buffer<int> fib_of[akm::gk_max_dims], fib_base[akm::gk_max_dims],
-      fib_len[akm::gk_max_dims], fib_map[akm::gk_max_dims];
+  akm::dev_buffer<int> elem;
+  akm::dev_buffer<int> nb_ptr[akm::gk_max_dims], nb_cell[akm::gk_max_dims],
+      nb_q[akm::gk_max_dims], nb_diag[akm::gk_max_dims];
   // chains
   std::vector<int> ch_nst_host;
   akm::dev_buffer<int> ch_term, ch_nst, ch_dim, ch_fill, ch_slot;
   std::vector<akm::dev_buffer<int>> stage_list; // per intermediate stage: chains
   std::vector<int> stage_count;
   int nchains = 0, nslots = 0, max_stages = 0;
+  int lanes = 1; // lanes per DOF row: 1 (coalesced tensors) or 8 (split lists)
   akm::dev_buffer<unsigned char> work;
   akm::dev_buffer<double> pc; // Jacobi diagonal (active DOFs)
   std::vector<int> c1_pntr_h, c1_indx_h, c1_diag_h;
@@ -239,15 +262,12 @@ static gk_args global_args(akm_matrix_s const &m)
   a.active_cells = g.active_cells;
   a.cells = g.cells;
   a.elem = g.elem.data();
-  a.c1_pntr = g.c1_pntr.data();
-  a.c1_indx = g.c1_indx.data();
-  a.c1_diag = g.c1_diag.data();
   for (int k = 0; k < m.d; k++)
   {
-    a.fib_of[k] = g.fib_of[k].data();
-    a.fib_base[k] = g.fib_base[k].data();
-    a.fib_len[k] = g.fib_len[k].data();
-    a.fib_map[k] = g.fib_map[k].data();
+    a.nb_ptr[k] = g.nb_ptr[k].data();
+    a.nb_cell[k] = g.nb_cell[k].data();
+    a.nb_q[k] = g.nb_q[k].data();
+    a.nb_diag[k] = g.nb_diag[k].data();
   }
   a.ch_term = g.ch_term.data();
   a.ch_nst = g.ch_nst.data();
@@ -277,17 +297,27 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
   const gk_args a = global_args(m);
   const P *vA = reinterpret_cast<const P *>(m.vA.data());
   P *work = reinterpret_cast<P *>(g.work.data());
+  const int lanes = (g.lanes == 8) ? 8 : 1;
   for (int st = 0; st + 1 < g.max_stages; st++)
   {
     const int nl = g.stage_count[st];
     if (nl == 0)
       continue;
-    gk_stage<P><<<gk_blocks(g.cells * m.N * nl), gk_threads, 0, m.stream>>>(
-        a, vA, g.stage_list[st].data(), nl, st, x, work);
+    const int blocks = gk_blocks(g.cells * m.N * nl * lanes);
+    if (lanes == 8)
+      gk_stage<P, 8><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl, st,
+                                                           x, work);
+    else
+      gk_stage<P, 1><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl, st,
+                                                           x, work);
     AKM_TRY(check_launch("global kron stage"));
   }
-  gk_final<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, vA, x, work, alpha, beta, y,
-                                                             gamma, xg, pc, jdt);
+  if (lanes == 8)
+    gk_final<P, 8><<<gk_blocks(dofs * 8), gk_threads, 0, m.stream>>>(a, vA, x, work, alpha, beta,
+                                                                     y, gamma, xg, pc, jdt);
+  else
+    gk_final<P, 1><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, vA, x, work, alpha, beta, y,
+                                                                 gamma, xg, pc, jdt);
   return check_launch("global kron final stage");
 }
 
@@ -351,6 +381,43 @@ int global_update(akm_matrix_s &m, const void *vA)
 }
 
 int64_t global_flops(akm_matrix_s const &m) { return m.gk ? m.gk->flops : 0; }
+
+int global_lanes(akm_matrix_s const &m) { return m.gk ? m.gk->lanes : 0; }
+
+//! Times both lane layouts on the operator itself (y is overwritten) and
+//! keeps the faster.
+int global_autotune(akm_matrix_s &m, const void *x, void *y)
+{
+  akm_global_s &g = *m.gk;
+  hipEvent_t e0, e1;
+  AKM_HIP(hipEventCreate(&e0));
+  AKM_HIP(hipEventCreate(&e1));
+  float best = -1;
+  int best_lanes = 1;
+  int rc = AKM_SUCCESS;
+  for (int lanes : {1, 8})
+  {
+    g.lanes = lanes;
+    rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
+    if (rc != AKM_SUCCESS)
+      break;
+    (void)hipEventRecord(e0, m.stream);
+    for (int r = 0; r < 5 && rc == AKM_SUCCESS; r++)
+      rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
+    (void)hipEventRecord(e1, m.stream);
+    float ms = 0;
+    if (rc == AKM_SUCCESS && hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (best < 0 || ms < best))
+    {
+      best = ms;
+      best_lanes = lanes;
+    }
+  }
+  g.lanes = best_lanes;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
 
 //! Ancestors of a 1D cell in the volume hierarchy: its parent chain down to
 //! cell 0 (the part of a connect_1d volume row before the diagonal,
@@ -423,18 +490,15 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
 
   // connect_1d (full hierarchy), the pattern of every 1D factor
   AKM_TRY(build_connect_1d(m.max_level, true, g->c1_pntr_h, g->c1_indx_h, g->c1_diag_h));
-  AKM_TRY(g->c1_pntr.upload(g->c1_pntr_h.data(), int64_t(g->c1_pntr_h.size())));
-  AKM_TRY(g->c1_indx.upload(g->c1_indx_h.data(), int64_t(g->c1_indx_h.size())));
-  AKM_TRY(g->c1_diag.upload(g->c1_diag_h.data(), int64_t(g->c1_diag_h.size())));
 
-  // fibres per dimension: cells equal in every other 1D index; table from
-  // the 1D index in dimension k to the cell
+  // per dimension: the fibres (cells equal in every other 1D index) and for
+  // every cell its connected cells of the fibre, 1D index ascending
   std::vector<int64_t> nnz_dim(d, 0); // DOF-level pattern entries per dimension
   for (int k = 0; k < d; k++)
   {
     std::map<std::vector<int>, int> fib;
-    std::vector<int> fib_of(g->cells), fib_len;
-    std::vector<std::vector<std::pair<int, int>>> members;
+    std::vector<int> fib_of(g->cells);
+    std::vector<std::map<int, int>> members; // 1D index -> cell
     std::vector<int> other(d - 1);
     for (int64_t c = 0; c < g->cells; c++)
     {
@@ -448,33 +512,34 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
         members.emplace_back();
       }
       fib_of[c] = it->second;
-      members[it->second].emplace_back(g->elem_host[c * d + k], static_cast<int>(c));
+      members[it->second].emplace(g->elem_host[c * d + k], static_cast<int>(c));
     }
-    std::vector<int> base(members.size()), len(members.size()), map;
-    for (size_t f = 0; f < members.size(); f++)
+    std::vector<int> ptr(g->cells + 1, 0), ncell, nq, ndiag(g->cells, -1);
+    for (int64_t c = 0; c < g->cells; c++)
     {
-      int mx = 0;
-      for (auto const &p : members[f])
-        mx = std::max(mx, p.first + 1);
-      base[f] = static_cast<int>(map.size());
-      len[f] = mx;
-      if (map.size() + size_t(mx) > (size_t(1) << 31) - 1)
-        return set_error(AKM_ERR_UNSUPPORTED, "global kron: fibre tables too large");
-      map.resize(map.size() + mx, -1);
-      for (auto const &p : members[f])
-        map[base[f] + p.first] = p.second;
-      for (auto const &p : members[f])
-        for (int q = g->c1_pntr_h[p.first]; q < g->c1_pntr_h[p.first + 1]; q++)
-        {
-          const int c1 = g->c1_indx_h[q];
-          if (c1 < mx && map[base[f] + c1] >= 0)
-            nnz_dim[k] += int64_t(m.n) * m.N;
-        }
+      const int r1 = g->elem_host[c * d + k];
+      const auto &mem = members[fib_of[c]];
+      for (int q = g->c1_pntr_h[r1]; q < g->c1_pntr_h[r1 + 1]; q++)
+      {
+        auto it = mem.find(g->c1_indx_h[q]);
+        if (it == mem.end())
+          continue;
+        if (q == g->c1_diag_h[r1])
+          ndiag[c] = static_cast<int>(ncell.size());
+        ncell.push_back(it->second);
+        nq.push_back(q);
+      }
+      if (ndiag[c] < 0 || ncell[ndiag[c]] != c)
+        return set_error(AKM_ERR_INVALID, "global kron: connect_1d row without its diagonal");
+      if (ncell.size() > (size_t(1) << 31) - 1)
+        return set_error(AKM_ERR_UNSUPPORTED, "global kron: connection lists too large");
+      ptr[c + 1] = static_cast<int>(ncell.size());
+      nnz_dim[k] += int64_t(ptr[c + 1] - ptr[c]) * m.n * m.N;
     }
-    AKM_TRY(g->fib_of[k].upload(fib_of.data(), int64_t(fib_of.size())));
-    AKM_TRY(g->fib_base[k].upload(base.data(), int64_t(base.size())));
-    AKM_TRY(g->fib_len[k].upload(len.data(), int64_t(len.size())));
-    AKM_TRY(g->fib_map[k].upload(map.data(), int64_t(map.size())));
+    AKM_TRY(g->nb_ptr[k].upload(ptr.data(), int64_t(ptr.size())));
+    AKM_TRY(g->nb_cell[k].upload(ncell.data(), int64_t(ncell.size())));
+    AKM_TRY(g->nb_q[k].upload(nq.data(), int64_t(nq.size())));
+    AKM_TRY(g->nb_diag[k].upload(ndiag.data(), int64_t(ndiag.size())));
   }
 
   // chains (kronmult::permutes, src/device/asgard_kronmult.hpp:152-220, and

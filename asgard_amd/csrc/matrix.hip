@@ -577,9 +577,10 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
     return set_error(AKM_ERR_INVALID, "autotune: null argument");
   if (mat->mode == mode_global)
   {
-    // one kernel shape: nothing to choose
+    // lanes per DOF row (reported in waves_per_row); no LDS tables
+    AKM_TRY(global_autotune(*mat, x, y));
     if (waves_per_row)
-      *waves_per_row = 0;
+      *waves_per_row = global_lanes(*mat);
     if (lds_tables)
       *lds_tables = 0;
     return AKM_SUCCESS;

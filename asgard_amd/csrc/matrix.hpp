@@ -81,6 +81,10 @@ int global_apply(akm_matrix_s &m, double alpha, const void *x, double beta, void
 int global_update(akm_matrix_s &m, const void *vA);
 //! Global operator: flops per apply as the reference counts them.
 int64_t global_flops(akm_matrix_s const &m);
+//! Global operator: lanes per DOF row in use (1 or 8).
+int global_lanes(akm_matrix_s const &m);
+//! Global operator: times the lane layouts on the operator, keeps the faster.
+int global_autotune(akm_matrix_s &m, const void *x, void *y);
 
 //! Kernel arguments of an FP64 matrix (vectors and scalars left empty).
 kron_args<double> kron_args_f64(akm_matrix_s const &m);

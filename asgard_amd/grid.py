@@ -151,3 +151,19 @@ def build_dense_terms(d, n, T, num_1d_blocks, ops):
             parts.append(b.transpose(2, 0, 3, 1).reshape(-1))
         terms.append(np.concatenate(parts))
     return terms
+
+
+def identity_mask(ops, num_terms, d):
+    """active[t][k] for akm_create_global_grid: 0 where the (t, k) 1D operator
+    is exactly the identity matrix, 1 elsewhere (the reference decides this
+    from the term's structure: a pure mass term is the identity in the
+    orthonormal basis, src/asgard_kronmult_matrix.cpp:1175-1184)."""
+    out = []
+    for t in range(num_terms):
+        row = []
+        for k in range(d):
+            a = np.asarray(ops[t * d + k])
+            row.append(0 if a.shape[0] == a.shape[1] and np.array_equal(a, np.eye(a.shape[0]))
+                       else 1)
+        out.append(row)
+    return out
