@@ -17,15 +17,17 @@
 // tensor per cell, dimension 0 slowest), padded cells appended after the
 // active ones.  For every dimension k and cell the host lists the connected
 // cells of its fibre (cells equal in all other 1D indices) with their
-// connect_1d positions.  A one-dimensional product in dimension k is one
-// lane per (chain, DOF), the lanes of a cell reading each connected cell's
-// tensor together, or (for small operators, where the longest lists set the
-// time) a group of 8 lanes per DOF that split the list and sum by a fixed
-// shuffle tree; akm_autotune times both.  The factor values are the same
-// connect_1d blocks (vA) the pair operator uses.  All chains of a
-// term set advance together, one launch per stage; the last stage of every
-// chain, the sum over chains and alpha/beta/gamma (and optionally the
-// reference's Jacobi scaling) are one launch.
+// connect_1d positions.  A one-dimensional product in dimension k runs in
+// one of three layouts, which akm_autotune times: one lane per (chain, DOF),
+// the lanes of a cell reading each connected cell's tensor together; a group
+// of 8 lanes per DOF that split the list and sum by a fixed shuffle tree;
+// or one wavefront per (chain, cell) whose lane groups split the list.
+// Cells are taken fibre by fibre, so the cells reading the same tensors run
+// together.  The factor values are the same connect_1d blocks (vA) the pair
+// operator uses.  All chains advance together, one launch per stage, each
+// chain in its own ping-pong slot; one more launch sums the chains in chain
+// order and applies alpha/beta/gamma (and optionally the reference's Jacobi
+// scaling).
 #include "akm_internal.hpp"
 #include "grid.hpp"
 #include "matrix.hpp"
@@ -59,13 +61,14 @@ struct gk_args
   const int *nb_cell[gk_max_dims];
   const int *nb_q[gk_max_dims];
   const int *nb_diag[gk_max_dims]; // position of the cell itself
+  const int *order[gk_max_dims];   // cells fibre by fibre (processing order)
   const int *ch_term;  // chain -> term
   const int *ch_nst;   // chain -> number of stages
   const int *ch_dim;   // chain -> dims of its stages (gk_max_dims each)
   const int *ch_fill;  // chain -> fills of its stages
-  const int *ch_slot;  // chain -> work-buffer slot (-1 without intermediates)
+  const int *ch_slot;  // chain -> work-buffer slot (-1 for identity-only terms)
   int nchains;
-  int nslots;          // chains with intermediate stages
+  int nslots;          // chains with at least one stage
   int64_t stride;      // work-buffer slot size (padded DOFs)
 };
 
@@ -129,8 +132,9 @@ __device__ __forceinline__ P gk_group_sum(P v)
   return v;
 }
 
-//! Intermediate stage `st` of every chain with more than st + 1 stages; one
-//! group of LANES lanes per (chain, padded DOF).
+//! Stage `st` of every chain with more than st stages; one group of LANES
+//! lanes per (chain, padded DOF), cells taken fibre by fibre so that the
+//! cells reading the same connected tensors run together.
 template<typename P, int LANES>
 __global__ void __launch_bounds__(gk_threads)
     gk_stage(gk_args a, const P *__restrict__ vA, const int *__restrict__ list, int nlist, int st,
@@ -143,10 +147,12 @@ __global__ void __launch_bounds__(gk_threads)
        gi += int64_t(gk_threads / LANES) * gridDim.x)
   {
     const int li = static_cast<int>(gi / dofs);
-    const int64_t i = gi - int64_t(li) * dofs;
+    const int64_t pos = gi - int64_t(li) * dofs;
     const int c = list[li];
     const int slot = a.ch_slot[c];
     const int k = a.ch_dim[c * gk_max_dims + st];
+    const int64_t pcell = pos / a.N;
+    const int64_t i = int64_t(a.order[k][pcell]) * a.N + (pos - pcell * a.N);
     const int fill = a.ch_fill[c * gk_max_dims + st];
     const P *in = (st == 0) ? x : work + (int64_t((st - 1) & 1) * a.nslots + slot) * a.stride;
     const P v = gk_group_sum<P, LANES>(
@@ -156,48 +162,146 @@ __global__ void __launch_bounds__(gk_threads)
   }
 }
 
-//! Last stage of every chain, summed over the chains (each lane in chain
-//! order, then the group tree): y = (alpha * sum + beta*y + gamma*xg) /
-//! (1 - jdt * pc) (no division when pc is null).
-template<typename P, int LANES>
+//! Cell layout: one wavefront per (chain, cell).  Lane (g, e), e < N, g <
+//! 64/N, takes every (64/N)-th connection of the cell's list starting at g;
+//! the lanes of one g read a connected cell's tensor together, and the list
+//! itself is loaded 64 entries at a time and passed between lanes.
+//! Returns this lane's partial sum for entry e (0 on idle lanes).
+template<typename P>
+__device__ __forceinline__ P gk_cell_part(gk_args const &a, const P *__restrict__ vA, int t, int k,
+                                          int fill, const P *__restrict__ in, bool in_active_only,
+                                          int64_t cell, int lane)
+{
+  const int N = a.N, n = a.n, d = a.d;
+  const int G = wave_size / N;
+  const int g = lane / N;
+  const int e = lane - g * N;
+  int stride_k = 1;
+  for (int j = k + 1; j < d; j++)
+    stride_k *= n;
+  const int s = (e / stride_k) % n;
+  const int e0 = e - s * stride_k;
+  const int dg = a.nb_diag[k][cell];
+  const int lo = (fill == gk_upper) ? dg : a.nb_ptr[k][cell];
+  const int hi = (fill == gk_lower) ? dg + 1 : a.nb_ptr[k][cell + 1];
+  const int64_t nn = int64_t(n) * n;
+  const int64_t tstride = int64_t(d) * nn;
+  const int64_t cstride = tstride * a.T;
+  const P *Ab = vA + int64_t(t) * tstride + k * nn + s;
+  P acc = 0;
+  for (int base = lo; base < hi; base += wave_size)
+  {
+    const int pl = base + lane;
+    const int my_cj = (pl < hi) ? a.nb_cell[k][pl] : 0;
+    const int my_q = (pl < hi) ? a.nb_q[k][pl] : 0;
+    const int cnt = min(wave_size, hi - base);
+    // the same trip count on every lane: the shuffles run with the whole
+    // wavefront active (a shuffle from an inactive lane reads zero)
+    const int iters = (cnt + G - 1) / G;
+    for (int it = 0; it < iters; it++)
+    {
+      const int r = it * G + g;
+      const int rr = min(r, wave_size - 1);
+      const int cj = __shfl(my_cj, rr, wave_size);
+      const int q = __shfl(my_q, rr, wave_size);
+      if (g < G && r < cnt && !(in_active_only && cj >= a.active_cells))
+      {
+        int jlo = 0, jhi = n;
+        if (base + r == dg)
+        {
+          if (fill == gk_upper)
+            jlo = s;
+          else if (fill == gk_lower)
+            jhi = s;
+        }
+        const P *A = Ab + int64_t(q) * cstride;
+        const P *xin = in + int64_t(cj) * N + e0;
+        for (int j = jlo; j < jhi; j++)
+          acc = fma(A[j * n], xin[j * stride_k], acc);
+      }
+    }
+  }
+  return acc;
+}
+
+//! Orders the LDS accesses of one wavefront (no workgroup barrier needed).
+__device__ __forceinline__ void gk_wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+//! Sums the wavefront's partials over g in order into lanes g = 0 (LDS
+//! slice of wave_size values owned by this wavefront).
+template<typename P>
+__device__ __forceinline__ P gk_cell_reduce(P v, P *red, int lane, int N)
+{
+  red[lane] = v;
+  gk_wave_sync();
+  P sum = 0;
+  if (lane < N)
+    for (int g = 0; g < wave_size / N; g++)
+      sum += red[g * N + lane];
+  gk_wave_sync();
+  return sum;
+}
+
+//! Stage `st` in the cell layout.
+template<typename P>
 __global__ void __launch_bounds__(gk_threads)
-    gk_final(gk_args a, const P *__restrict__ vA, const P *__restrict__ x, const P *work, P alpha,
-             P beta, P *y, P gamma, const P *xg, const double *pc, double jdt)
+    gk_cell_stage(gk_args a, const P *__restrict__ vA, const int *__restrict__ list, int nlist,
+                  int st, const P *__restrict__ x, P *work)
+{
+  __shared__ P red[gk_threads];
+  const int lane = threadIdx.x % wave_size;
+  P *wred = red + (threadIdx.x / wave_size) * wave_size;
+  const int64_t items = a.cells * nlist;
+  for (int64_t w = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / wave_size; w < items;
+       w += int64_t(gk_threads / wave_size) * gridDim.x)
+  {
+    const int li = static_cast<int>(w / a.cells);
+    const int c = list[li];
+    const int slot = a.ch_slot[c];
+    const int k = a.ch_dim[c * gk_max_dims + st];
+    const int64_t cell = a.order[k][w - int64_t(li) * a.cells];
+    const P *in = (st == 0) ? x : work + (int64_t((st - 1) & 1) * a.nslots + slot) * a.stride;
+    const P v = gk_cell_reduce(gk_cell_part<P>(a, vA, a.ch_term[c], k,
+                                               a.ch_fill[c * gk_max_dims + st], in, st == 0,
+                                               cell, lane),
+                               wred, lane, a.N);
+    if (lane < a.N)
+      work[(int64_t(st & 1) * a.nslots + slot) * a.stride + cell * a.N + lane] = v;
+  }
+}
+
+//! y = (alpha * sum over chains of the chain's last stage (x for a term of
+//! identity factors), in chain order, + beta*y + gamma*xg) / (1 - jdt * pc)
+//! (no division when pc is null).
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gk_sum(gk_args a, const P *__restrict__ x, const P *__restrict__ work, P alpha, P beta, P *y,
+           P gamma, const P *xg, const double *pc, double jdt)
 {
   const int64_t dofs = a.active_cells * a.N;
-  const int g = threadIdx.x % LANES;
-  for (int64_t i = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / LANES; i < dofs;
-       i += int64_t(gk_threads / LANES) * gridDim.x)
+  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < dofs;
+       i += int64_t(gk_threads) * gridDim.x)
   {
     P sum = 0;
     for (int c = 0; c < a.nchains; c++)
     {
       const int ns = a.ch_nst[c];
-      if (ns == 0)
-      {
-        if (g == 0)
-          sum += x[i]; // a term of identity factors only
-        continue;
-      }
-      const int st = ns - 1;
-      const P *in = (st == 0) ? x
-                              : work + (int64_t((st - 1) & 1) * a.nslots + a.ch_slot[c]) *
-                                           a.stride;
-      sum += gk_row_part<P, LANES>(a, vA, a.ch_term[c], a.ch_dim[c * gk_max_dims + st],
-                                   a.ch_fill[c * gk_max_dims + st], in, st == 0, i, g);
+      sum += (ns == 0) ? x[i]
+                       : work[(int64_t((ns - 1) & 1) * a.nslots + a.ch_slot[c]) * a.stride + i];
     }
-    sum = gk_group_sum<P, LANES>(sum);
-    if (g == 0)
-    {
-      P out = alpha * sum;
-      if (beta != P{0})
-        out += beta * y[i];
-      if (gamma != P{0})
-        out += gamma * xg[i];
-      if (pc != nullptr)
-        out = static_cast<P>(static_cast<double>(out) / (1.0 - jdt * pc[i]));
-      y[i] = out;
-    }
+    P out = alpha * sum;
+    if (beta != P{0})
+      out += beta * y[i];
+    if (gamma != P{0})
+      out += gamma * xg[i];
+    if (pc != nullptr)
+      out = static_cast<P>(static_cast<double>(out) / (1.0 - jdt * pc[i]));
+    y[i] = out;
   }
 }
 
@@ -233,14 +337,16 @@ struct akm_global_s
   std::vector<int> elem_host; // padded cells x d
   akm::dev_buffer<int> elem;
   akm::dev_buffer<int> nb_ptr[akm::gk_max_dims], nb_cell[akm::gk_max_dims],
-      nb_q[akm::gk_max_dims], nb_diag[akm::gk_max_dims];
+      nb_q[akm::gk_max_dims], nb_diag[akm::gk_max_dims], order[akm::gk_max_dims];
   // chains
   std::vector<int> ch_nst_host;
   akm::dev_buffer<int> ch_term, ch_nst, ch_dim, ch_fill, ch_slot;
-  std::vector<akm::dev_buffer<int>> stage_list; // per intermediate stage: chains
+  std::vector<akm::dev_buffer<int>> stage_list; // per stage: the chains that have it
   std::vector<int> stage_count;
   int nchains = 0, nslots = 0, max_stages = 0;
-  int lanes = 1; // lanes per DOF row: 1 (coalesced tensors) or 8 (split lists)
+  // lanes per DOF row: 1 (coalesced tensors), 8 (split lists), or 0: one
+  // wavefront per cell (n^d <= 64)
+  int lanes = 1;
   akm::dev_buffer<unsigned char> work;
   akm::dev_buffer<double> pc; // Jacobi diagonal (active DOFs)
   std::vector<int> c1_pntr_h, c1_indx_h, c1_diag_h;
@@ -268,6 +374,7 @@ static gk_args global_args(akm_matrix_s const &m)
     a.nb_cell[k] = g.nb_cell[k].data();
     a.nb_q[k] = g.nb_q[k].data();
     a.nb_diag[k] = g.nb_diag[k].data();
+    a.order[k] = g.order[k].data();
   }
   a.ch_term = g.ch_term.data();
   a.ch_nst = g.ch_nst.data();
@@ -297,14 +404,18 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
   const gk_args a = global_args(m);
   const P *vA = reinterpret_cast<const P *>(m.vA.data());
   P *work = reinterpret_cast<P *>(g.work.data());
-  const int lanes = (g.lanes == 8) ? 8 : 1;
-  for (int st = 0; st + 1 < g.max_stages; st++)
+  const int lanes = g.lanes;
+  for (int st = 0; st < g.max_stages; st++)
   {
     const int nl = g.stage_count[st];
     if (nl == 0)
       continue;
-    const int blocks = gk_blocks(g.cells * m.N * nl * lanes);
-    if (lanes == 8)
+    const int blocks = gk_blocks(lanes == 0 ? g.cells * nl * wave_size
+                                            : g.cells * m.N * nl * lanes);
+    if (lanes == 0)
+      gk_cell_stage<P><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl,
+                                                            st, x, work);
+    else if (lanes == 8)
       gk_stage<P, 8><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl, st,
                                                            x, work);
     else
@@ -312,13 +423,9 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
                                                            x, work);
     AKM_TRY(check_launch("global kron stage"));
   }
-  if (lanes == 8)
-    gk_final<P, 8><<<gk_blocks(dofs * 8), gk_threads, 0, m.stream>>>(a, vA, x, work, alpha, beta,
-                                                                     y, gamma, xg, pc, jdt);
-  else
-    gk_final<P, 1><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, vA, x, work, alpha, beta, y,
-                                                                 gamma, xg, pc, jdt);
-  return check_launch("global kron final stage");
+  gk_sum<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, x, work, alpha, beta, y, gamma, xg,
+                                                          pc, jdt);
+  return check_launch("global kron sum");
 }
 
 int global_apply(akm_matrix_s &m, double alpha, const void *x, double beta, void *y,
@@ -384,6 +491,20 @@ int64_t global_flops(akm_matrix_s const &m) { return m.gk ? m.gk->flops : 0; }
 
 int global_lanes(akm_matrix_s const &m) { return m.gk ? m.gk->lanes : 0; }
 
+int global_set_layout(akm_matrix_s &m, int layout)
+{
+  if (!m.gk)
+    return set_error(AKM_ERR_INVALID, "not a global kron matrix");
+  if (layout == 0)
+    return AKM_SUCCESS;
+  if (layout != 1 && layout != 2 && layout != 8)
+    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2 or 8", layout);
+  if (layout == 2 && m.N > wave_size)
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: the cell layout needs n^d <= 64");
+  m.gk->lanes = (layout == 2) ? 0 : layout;
+  return AKM_SUCCESS;
+}
+
 //! Times both lane layouts on the operator itself (y is overwritten) and
 //! keeps the faster.
 int global_autotune(akm_matrix_s &m, const void *x, void *y)
@@ -395,8 +516,10 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
   float best = -1;
   int best_lanes = 1;
   int rc = AKM_SUCCESS;
-  for (int lanes : {1, 8})
+  for (int lanes : {1, 8, 0})
   {
+    if (lanes == 0 && m.N > wave_size)
+      continue;
     g.lanes = lanes;
     rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
     if (rc != AKM_SUCCESS)
@@ -540,6 +663,13 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
     AKM_TRY(g->nb_cell[k].upload(ncell.data(), int64_t(ncell.size())));
     AKM_TRY(g->nb_q[k].upload(nq.data(), int64_t(nq.size())));
     AKM_TRY(g->nb_diag[k].upload(ndiag.data(), int64_t(ndiag.size())));
+    // processing order: fibre by fibre (fibres in key order), 1D index ascending
+    std::vector<int> ord;
+    ord.reserve(g->cells);
+    for (auto const &kv : fib)
+      for (auto const &mc : members[kv.second])
+        ord.push_back(mc.second);
+    AKM_TRY(g->order[k].upload(ord.data(), int64_t(ord.size())));
   }
 
   // chains (kronmult::permutes, src/device/asgard_kronmult.hpp:152-220, and
@@ -600,7 +730,7 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
   for (int c = 0; c < g->nchains; c++)
   {
     g->max_stages = std::max(g->max_stages, ch_nst[c]);
-    ch_slot.push_back(ch_nst[c] >= 2 ? nslots++ : -1);
+    ch_slot.push_back(ch_nst[c] >= 1 ? nslots++ : -1);
   }
   g->nslots = nslots;
   g->ch_nst_host = ch_nst;
@@ -612,14 +742,14 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
     AKM_TRY(g->ch_fill.upload(ch_fill.data(), int64_t(ch_fill.size())));
     AKM_TRY(g->ch_slot.upload(ch_slot.data(), int64_t(ch_slot.size())));
   }
-  // chains per intermediate stage (those with a later stage)
-  g->stage_list.resize(std::max(0, g->max_stages - 1));
-  g->stage_count.assign(std::max(0, g->max_stages - 1), 0);
-  for (int st = 0; st + 1 < g->max_stages; st++)
+  // chains per stage (those with more than st stages)
+  g->stage_list.resize(g->max_stages);
+  g->stage_count.assign(g->max_stages, 0);
+  for (int st = 0; st < g->max_stages; st++)
   {
     std::vector<int> list;
     for (int c = 0; c < g->nchains; c++)
-      if (ch_nst[c] >= st + 2)
+      if (ch_nst[c] >= st + 1)
         list.push_back(c);
     g->stage_count[st] = static_cast<int>(list.size());
     if (!list.empty())

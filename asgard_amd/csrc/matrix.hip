@@ -536,6 +536,8 @@ akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
 {
   if (mat == nullptr)
     return set_error(AKM_ERR_INVALID, "null matrix");
+  if (mat->mode == mode_global)
+    return global_set_layout(*mat, waves_per_row);
   if (waves_per_row != 0 && waves_per_row != 1 && waves_per_row != 2 && waves_per_row != 4 &&
       waves_per_row != 8)
     return set_error(AKM_ERR_INVALID, "waves per row must be 0, 1, 2, 4 or 8 (got %d)",
@@ -577,10 +579,10 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
     return set_error(AKM_ERR_INVALID, "autotune: null argument");
   if (mat->mode == mode_global)
   {
-    // lanes per DOF row (reported in waves_per_row); no LDS tables
+    // row layout (reported in waves_per_row as akm_set_tuning takes it)
     AKM_TRY(global_autotune(*mat, x, y));
     if (waves_per_row)
-      *waves_per_row = global_lanes(*mat);
+      *waves_per_row = (global_lanes(*mat) == 0) ? 2 : global_lanes(*mat);
     if (lds_tables)
       *lds_tables = 0;
     return AKM_SUCCESS;

@@ -85,6 +85,9 @@ int64_t global_flops(akm_matrix_s const &m);
 int global_lanes(akm_matrix_s const &m);
 //! Global operator: times the lane layouts on the operator, keeps the faster.
 int global_autotune(akm_matrix_s &m, const void *x, void *y);
+//! Global operator: row layout 1 (lane per DOF), 8 (8 lanes per DOF), 2
+//! (wavefront per cell), 0 (unchanged).
+int global_set_layout(akm_matrix_s &m, int layout);
 
 //! Kernel arguments of an FP64 matrix (vectors and scalars left empty).
 kron_args<double> kron_args_f64(akm_matrix_s const &m);

@@ -175,8 +175,11 @@ akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_
  * 8), the LDS bytes a workgroup may hold for the row's factor tables
  * (-1 = automatic, 0 = read factors from global memory) and whether to
  * force the block-wide LDS kernel (for tests).  Takes effect at the next
- * apply.  No counterpart in the reference (its launch shape is fixed at
- * build time, CMakeLists.txt:231-232). */
+ * apply.  On a global operator only waves_per_row counts and picks the row
+ * layout: 1 = one lane per DOF, 8 = eight lanes per DOF, 2 = one wavefront
+ * per cell (n^d <= 64), 0 = unchanged; akm_autotune times them and reports
+ * its choice the same way.  No counterpart in the reference (its launch
+ * shape is fixed at build time, CMakeLists.txt:231-232). */
 akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
                           int force_generic);
 

@@ -54,13 +54,18 @@ CASES = [dict(d=2, n=2, L=4, T=2), dict(d=3, n=2, L=4, T=2), dict(d=3, n=3, L=3,
          dict(d=4, n=2, L=4, T=1, refine=0.2), dict(d=2, n=3, L=4, T=2, refine=0.4)]
 
 
+LAYOUTS = (1, 8, 2)  # lane per DOF, 8 lanes per DOF, wavefront per cell
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("spec", CASES, ids=lambda s: "-".join(f"{k}{v}" for k, v in s.items()))
-def test_global_parity(spec):
+def test_global_parity(spec, layout):
     c = grid_case(spec["d"], spec["n"], spec["L"], spec["T"], seed=3,
                   refine=spec.get("refine", 0.0))
     ref = gk.apply(c["d"], c["n"], c["table"], c["L"], c["T"], c["vA"], 1.5, c["x"], -0.5,
                    c["y"])
     m = global_matrix(c)
+    m.set_tuning(waves_per_row=layout)
     y = dev(c["y"])
     m.apply(1.5, dev(c["x"]), -0.5, y)
     assert rel(host(y), ref) < TOL64
@@ -70,6 +75,7 @@ def test_global_parity(spec):
     assert np.array_equal(host(y), host(y2))
     # FP32 operator
     m32 = global_matrix(c, np.float32)
+    m32.set_tuning(waves_per_row=layout)
     y32 = dev(c["y"].astype(np.float32))
     m32.apply(1.5, dev(c["x"].astype(np.float32)), -0.5, y32)
     assert rel(host(y32), ref) < TOL32
@@ -148,8 +154,27 @@ def test_update_coefficients_and_jacobi():
     assert rel(m.jacobi(), gk.jacobi(3, 2, c["table"], 4, 2, vA2)) < 1e-14
 
 
+def test_layout_choice():
+    c = grid_case(3, 2, 4, 2, seed=14)
+    m = global_matrix(c)
+    x = dev(c["x"])
+    layout = m.autotune(x, dev(np.zeros_like(c["x"])))[0]
+    assert layout in LAYOUTS
+    y = dev(np.zeros_like(c["x"]))
+    m.apply(1.0, x, 0.0, y)
+    ref = gk.apply(3, 2, c["table"], 4, 2, c["vA"], 1.0, c["x"], 0.0, c["y"])
+    assert rel(host(y), ref) < TOL64
+    with pytest.raises(asgard_amd.AkmError):
+        m.set_tuning(waves_per_row=4)
+    big = grid_case(2, 9, 2, 1, seed=15)  # n^d = 81: no wavefront-per-cell layout
+    mb = global_matrix(big)
+    with pytest.raises(asgard_amd.AkmError):
+        mb.set_tuning(waves_per_row=2)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("d,n,L,T,dt", [(2, 2, 4, 2, 0.01), (3, 2, 3, 1, 0.02)])
-def test_jacobi_gmres_matches_dense_oracle(d, n, L, T, dt):
+def test_jacobi_gmres_matches_dense_oracle(d, n, L, T, dt, layout):
     c = grid_case(d, n, L, T, seed=21)
     size = c["x"].size
     A = np.zeros((size, size))
@@ -164,6 +189,7 @@ def test_jacobi_gmres_matches_dense_oracle(d, n, L, T, dt):
     xo, erro, itso = oracle.gmres_dense(E, b, x0=np.zeros(size), M=Mdiag, restart=20,
                                         max_outer=20, tol=1e-11)
     m = global_matrix(c)
+    m.set_tuning(waves_per_row=layout)
     xg = dev(np.zeros(size))
     info = asgard_amd.simple_gmres_euler(dt, m, xg, dev(b), 20, 20, 1e-11)
     assert abs(info.iterations - itso) <= 1
@@ -179,7 +205,9 @@ def test_same_operator_as_pairwise_at_bench_size():
     glob = global_matrix(c)
     x = dev(c["x"])
     y1 = dev(np.zeros_like(c["x"]))
-    y2 = dev(np.zeros_like(c["x"]))
     pair.apply(1.0, x, 0.0, y1)
-    glob.apply(1.0, x, 0.0, y2)
-    assert rel(host(y2), host(y1)) < 1e-12
+    for layout in LAYOUTS:
+        glob.set_tuning(waves_per_row=layout)
+        y2 = dev(np.zeros_like(c["x"]))
+        glob.apply(1.0, x, 0.0, y2)
+        assert rel(host(y2), host(y1)) < 1e-12, layout
