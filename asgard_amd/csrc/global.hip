@@ -165,14 +165,18 @@ __global__ void __launch_bounds__(gk_threads)
 //! Cell layout: one wavefront per (chain, cell).  Lane (g, e), e < N, g <
 //! 64/N, takes every (64/N)-th connection of the cell's list starting at g;
 //! the lanes of one g read a connected cell's tensor together, and the list
-//! itself is loaded 64 entries at a time and passed between lanes.
-//! Returns this lane's partial sum for entry e (0 on idle lanes).
-template<typename P>
+//! itself is loaded 64 entries at a time and passed between lanes.  Four
+//! connections are fetched before any is used (compile-time n = NT; NT = 0
+//! reads n at run time).  Returns this lane's partial sum for entry e (0 on
+//! idle lanes).
+template<typename P, int NT>
 __device__ __forceinline__ P gk_cell_part(gk_args const &a, const P *__restrict__ vA, int t, int k,
                                           int fill, const P *__restrict__ in, bool in_active_only,
                                           int64_t cell, int lane)
 {
-  const int N = a.N, n = a.n, d = a.d;
+  constexpr int U = 4;
+  const int N = a.N, d = a.d;
+  const int n = (NT > 0) ? NT : a.n;
   const int G = wave_size / N;
   const int g = lane / N;
   const int e = lane - g * N;
@@ -198,26 +202,54 @@ __device__ __forceinline__ P gk_cell_part(gk_args const &a, const P *__restrict_
     // the same trip count on every lane: the shuffles run with the whole
     // wavefront active (a shuffle from an inactive lane reads zero)
     const int iters = (cnt + G - 1) / G;
-    for (int it = 0; it < iters; it++)
+    for (int it = 0; it < iters; it += U)
     {
-      const int r = it * G + g;
-      const int rr = min(r, wave_size - 1);
-      const int cj = __shfl(my_cj, rr, wave_size);
-      const int q = __shfl(my_q, rr, wave_size);
-      if (g < G && r < cnt && !(in_active_only && cj >= a.active_cells))
+      int cjv[U], qv[U], jlov[U], jhiv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++)
       {
-        int jlo = 0, jhi = n;
-        if (base + r == dg)
+        const int r = (it + u) * G + g;
+        const int rr = min(r, wave_size - 1);
+        const int cj = __shfl(my_cj, rr, wave_size);
+        const int q = __shfl(my_q, rr, wave_size);
+        const bool ok =
+            (it + u < iters) && g < G && r < cnt && !(in_active_only && cj >= a.active_cells);
+        const bool diag = ok && (base + r == dg);
+        // idle entries read cell 0 / connection 0 (valid addresses) and add nothing
+        cjv[u] = ok ? cj : 0;
+        qv[u] = ok ? q : 0;
+        jlov[u] = (diag && fill == gk_upper) ? s : 0;
+        jhiv[u] = !ok ? 0 : ((diag && fill == gk_lower) ? s : n);
+      }
+      if constexpr (NT > 0)
+      {
+        P av[U][NT], xv[U][NT];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int j = 0; j < NT; j++)
+          {
+            av[u][j] = Ab[int64_t(qv[u]) * cstride + j * NT];
+            xv[u][j] = in[int64_t(cjv[u]) * N + e0 + j * stride_k];
+          }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int j = 0; j < NT; j++)
+          {
+            const P v = fma(av[u][j], xv[u][j], acc);
+            acc = (j >= jlov[u] && j < jhiv[u]) ? v : acc;
+          }
+      }
+      else
+      {
+        for (int u = 0; u < U; u++)
         {
-          if (fill == gk_upper)
-            jlo = s;
-          else if (fill == gk_lower)
-            jhi = s;
+          const P *A = Ab + int64_t(qv[u]) * cstride;
+          const P *xin = in + int64_t(cjv[u]) * N + e0;
+          for (int j = jlov[u]; j < jhiv[u]; j++)
+            acc = fma(A[j * n], xin[j * stride_k], acc);
         }
-        const P *A = Ab + int64_t(q) * cstride;
-        const P *xin = in + int64_t(cj) * N + e0;
-        for (int j = jlo; j < jhi; j++)
-          acc = fma(A[j * n], xin[j * stride_k], acc);
       }
     }
   }
@@ -248,7 +280,7 @@ __device__ __forceinline__ P gk_cell_reduce(P v, P *red, int lane, int N)
 }
 
 //! Stage `st` in the cell layout.
-template<typename P>
+template<typename P, int NT>
 __global__ void __launch_bounds__(gk_threads)
     gk_cell_stage(gk_args a, const P *__restrict__ vA, const int *__restrict__ list, int nlist,
                   int st, const P *__restrict__ x, P *work)
@@ -266,7 +298,7 @@ __global__ void __launch_bounds__(gk_threads)
     const int k = a.ch_dim[c * gk_max_dims + st];
     const int64_t cell = a.order[k][w - int64_t(li) * a.cells];
     const P *in = (st == 0) ? x : work + (int64_t((st - 1) & 1) * a.nslots + slot) * a.stride;
-    const P v = gk_cell_reduce(gk_cell_part<P>(a, vA, a.ch_term[c], k,
+    const P v = gk_cell_reduce(gk_cell_part<P, NT>(a, vA, a.ch_term[c], k,
                                                a.ch_fill[c * gk_max_dims + st], in, st == 0,
                                                cell, lane),
                                wred, lane, a.N);
@@ -413,8 +445,23 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
     const int blocks = gk_blocks(lanes == 0 ? g.cells * nl * wave_size
                                             : g.cells * m.N * nl * lanes);
     if (lanes == 0)
-      gk_cell_stage<P><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl,
-                                                            st, x, work);
+    {
+      const int *list = g.stage_list[st].data();
+      switch (m.n)
+      {
+      case 2:
+        gk_cell_stage<P, 2><<<blocks, gk_threads, 0, m.stream>>>(a, vA, list, nl, st, x, work);
+        break;
+      case 3:
+        gk_cell_stage<P, 3><<<blocks, gk_threads, 0, m.stream>>>(a, vA, list, nl, st, x, work);
+        break;
+      case 4:
+        gk_cell_stage<P, 4><<<blocks, gk_threads, 0, m.stream>>>(a, vA, list, nl, st, x, work);
+        break;
+      default:
+        gk_cell_stage<P, 0><<<blocks, gk_threads, 0, m.stream>>>(a, vA, list, nl, st, x, work);
+      }
+    }
     else if (lanes == 8)
       gk_stage<P, 8><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl, st,
                                                            x, work);
