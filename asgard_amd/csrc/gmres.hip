@@ -339,6 +339,33 @@ static int64_t rows_per_block_for(int64_t n)
   return std::max<int64_t>(256, (n + 1023) / 1024);
 }
 
+//! Arnoldi steps are enqueued one ahead of the host's convergence check when
+//! the operator apply takes less than this (milliseconds).
+constexpr double speculate_below_ms = 0.25;
+
+//! Runs f (which enqueues work on s) and measures it with HIP events.
+template<typename F>
+static int timed_on_stream(hipStream_t s, F &&f, double &ms)
+{
+  hipEvent_t e0, e1;
+  AKM_HIP(hipEventCreate(&e0));
+  AKM_HIP(hipEventCreate(&e1));
+  int rc = AKM_SUCCESS;
+  if (hipEventRecord(e0, s) != hipSuccess)
+    rc = set_error(AKM_ERR_HIP, "gmres: event record failed");
+  if (rc == AKM_SUCCESS)
+    rc = f();
+  float t = 0;
+  if (rc == AKM_SUCCESS &&
+      (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+       hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+    rc = set_error(AKM_ERR_HIP, "gmres: event timing failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ms = t;
+  return rc;
+}
+
 //! One restart cycle's device state and the kernel sequence of its steps.
 template<typename P>
 struct gm_cycle
@@ -445,6 +472,31 @@ struct gm_cycle
     return AKM_SUCCESS;
   }
 
+  //! Arnoldi steps 0, 1, ... until the estimate reaches `stop` or `limit`
+  //! steps ran; count = the steps taken.  With `speculate`, step j + 1 is
+  //! enqueued before step j's estimate is read, which hides the host round
+  //! trip behind the GPU work.  When step j ends the cycle the speculative
+  //! step's column, Hessenberg column and rotation are never used (the
+  //! update reads columns and g entries below count only).
+  int run_steps(int limit, std::function<int(const P *, P *)> const &op,
+                std::function<int(const P *, P *)> const *pre, P tol, P stop, P &est,
+                int &count, bool speculate)
+  {
+    count = 0;
+    bool ahead = false;
+    while (est > stop && count < limit)
+    {
+      if (!ahead)
+        AKM_TRY(launch_step(count, op, pre, tol));
+      ahead = speculate && count + 1 < limit;
+      if (ahead)
+        AKM_TRY(launch_step(count + 1, op, pre, tol));
+      AKM_TRY(wait_step(count, est));
+      ++count;
+    }
+    return AKM_SUCCESS;
+  }
+
   //! Waits for step j's state; est = its residual estimate.
   int wait_step(int j, P &est)
   {
@@ -486,20 +538,27 @@ static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int
 
   int total = 0, outer = 0;
   P inner_res = 0, outer_res = tol + 1;
+  bool speculate = false;
   while (outer_res > tol && outer < max_outer)
   {
     // r = b - (I - dt A) x into V_0
     AKM_HIP(hipMemcpyAsync(V, b, sizeof(P) * n, hipMemcpyDeviceToDevice, s));
-    AKM_TRY(apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x));
+    if (outer == 0)
+    {
+      double ms = 0;
+      AKM_TRY(timed_on_stream(s, [&]() { return apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x); },
+                              ms));
+      speculate = ms < speculate_below_ms;
+    }
+    else
+    {
+      AKM_TRY(apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x));
+    }
     ++total;
     AKM_TRY(cyc.start(tol, inner_res));
     int inner = 0;
-    while (inner_res > tol && inner < restart)
-    {
-      AKM_TRY(cyc.step(inner, op, nullptr, tol, inner_res));
-      ++total;
-      ++inner;
-    }
+    AKM_TRY(cyc.run_steps(restart, op, nullptr, tol, tol, inner_res, inner, speculate));
+    total += inner;
     AKM_TRY(cyc.finish(inner, x, false));
     ++outer;
     outer_res = inner_res;
@@ -568,19 +627,27 @@ static int gmres_mixed(akm_matrix_s &m64, akm_matrix_s *m32, akm_precond_s *pre,
   const double cycle_rel = (m32 != nullptr) ? inner_rel : 0.0;
 
   double beta = 0;
+  double first_ms = 0;
   if (m32 != nullptr)
   {
     // the first cycle only needs its residual to the accuracy of its own
     // target (inner_rel of it): b - x in FP64 and dt*A*x with the FP32 operator
-    AKM_TRY(op32(x, V));
+    AKM_TRY(timed_on_stream(s, [&]() { return op32(x, V); }, first_ms));
     gm_rsub<double><<<sblocks, gm_threads, 0, s>>>(n, b, V);
     AKM_TRY(check_launch("gmres residual"));
     AKM_TRY(cyc.start(0.0, beta));
   }
   else
   {
-    AKM_TRY(true_residual(beta));
+    AKM_HIP(hipMemcpyAsync(V, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    ++stats.applies_f64;
+    AKM_TRY(timed_on_stream(s, [&]() { return apply_device(m64, dt, x, 1.0, V, -1.0, x); },
+                            first_ms));
+    AKM_TRY(cyc.start(0.0, beta));
   }
+  // speculative steps pay off when a step's operator is cheap next to the
+  // host round trip; the preconditioner (if any) runs in every step too
+  const bool speculate = (pre == nullptr) && first_ms < speculate_below_ms;
   bool fallback = false;
   while (beta > tol_abs && stats.cycles < max_cycles)
   {
@@ -589,15 +656,9 @@ static int gmres_mixed(akm_matrix_s &m64, akm_matrix_s *m32, akm_precond_s *pre,
       AKM_HIP(hipMemcpyAsync(ws.xprev.data(), x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     const auto &op = fallback ? op64 : inner_op;
     const auto *pc = (fallback || pre == nullptr) ? nullptr : &precond;
-    // the host reads every step's estimate before the next step: a step
-    // costs a whole operator apply, far more than the round trip
     double est = beta;
     int inner = 0;
-    while (est > stop && inner < restart)
-    {
-      AKM_TRY(cyc.step(inner, op, pc, stop, est));
-      ++inner;
-    }
+    AKM_TRY(cyc.run_steps(restart, op, pc, stop, stop, est, inner, speculate && !fallback));
     AKM_TRY(cyc.finish(inner, x, pc != nullptr));
     stats.iterations += inner;
     if (fallback)

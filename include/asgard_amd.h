@@ -231,7 +231,9 @@ typedef struct
   int iterations;          /* Krylov iterations over all cycles */
   int cycles;              /* refinement (restart) cycles */
   int applies_f64;         /* FP64 operator applications (residuals included) */
-  int applies_f32;         /* FP32 operator applications */
+  int applies_f32;         /* FP32 operator applications; with a cheap operator
+                              (under 0.25 ms per apply) each cycle also runs one
+                              speculative step whose result is discarded */
   int precond_applies;     /* preconditioner applications */
   int fallback;            /* 1 if plain FP64 GMRES took over */
   int fallback_iterations; /* Krylov iterations of the plain FP64 fallback */

@@ -79,7 +79,9 @@ def test_fp32_refinement_reaches_fp64_accuracy(d, n, L, dt):
     assert st.converged == 1 and st.fallback == 0
     assert st.rel_residual <= 1e-10
     # the first cycle's residual uses the FP32 operator, every later one FP64
-    assert st.applies_f32 == st.iterations + 1 and st.iterations > 0
+    # one speculative (discarded) step per cycle when the operator is cheap
+    assert st.iterations + 1 <= st.applies_f32 <= st.iterations + 1 + st.cycles
+    assert st.iterations > 0
     assert st.applies_f64 == st.cycles
     xs = host(x)
     # the FP64 residual recomputed independently on the CPU
