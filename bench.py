@@ -236,6 +236,54 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
     return out
 
 
+def global_operator_applies(reps=20):
+    """Applies of the global Kronecker operator (the reference's default
+    mode) on the two solve workloads' operators and on the bench workload's
+    random factors; per apply the time, the layout the autotuner chose and
+    the algorithmic bytes (every stage reads and writes one padded vector
+    per chain, the chain sum reads one vector per chain and writes y)."""
+    import torch
+
+    import asgard_amd
+    from asgard_amd import grid
+
+    out = {}
+    cases = [("d4n2_adv_diff", "d4n2"), ("d6n2_vlasov", "d6n2"), ("d3n3_random", None)]
+    for label, key in cases:
+        if key is None:
+            d, n, L, T = 3, 3, 6, 3
+            table, vA, _ = make_workload(d, n, L, T)
+            act = None
+        else:
+            w = SOLVE_WORKLOADS[key]
+            d, n, L = w["d"], w["n"], w["L"]
+            table, _ = grid.element_table([L] * d)
+            ops = solve_operators(key)
+            T = len(ops) // d
+            vA = grid.build_vA(d, n, T, L, ops)
+            act = grid.identity_mask(ops, T, d)
+        rec = {}
+        for prec, dtype in (("fp64", np.float64), ("fp32", np.float32)):
+            m = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA.astype(dtype),
+                                                      dtype=dtype, active=act)
+            x = torch.from_numpy(np.random.default_rng(0).standard_normal(
+                m.output_size()).astype(dtype)).cuda()
+            y = torch.zeros_like(x)
+            layout = m.autotune(x, y)[0]
+            ms = m.time_apply(1.0, x, 0.0, y, reps)
+            pad, chains, stages = m.global_info()
+            dofs = m.output_size()
+            padded = dofs + pad * n ** d
+            esize = np.dtype(dtype).itemsize
+            moved = (chains * stages * 2 * padded + (chains + 1) * dofs) * esize
+            rec[prec] = {"us_per_apply": round(ms * 1e3, 2), "layout": layout,
+                         "chains": chains, "stages": stages,
+                         "algorithmic_bytes": int(moved),
+                         "achieved_GBps": round(moved / (ms * 1e-3) * 1e-9, 1)}
+        out[label] = rec
+    return out
+
+
 def batch_systems(count=64, seed=0):
     """The batched-solve workload: `count` implicit steps, d ~ U{2..5}, n in
     {2, 3}, each on the |l|_1 <= 5 sparse grid with 10-30% of its leaf cells
@@ -360,11 +408,16 @@ def main():
                          + ", ".join(list(SOLVE_WORKLOADS) + ["batch64"]) + "; 'none' to skip)")
     ap.add_argument("--solve-only", action="store_true",
                     help="print only the solve comparisons (one JSON line each)")
+    ap.add_argument("--global-only", action="store_true",
+                    help="print only the global-operator applies (one JSON line)")
     ap.add_argument("--samples", type=int, default=200_000,
                     help="per-cell training samples of the learned preconditioner")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.global_only:
+        print(json.dumps({"global_operator_applies": global_operator_applies()}), flush=True)
         return
     solves = [] if args.solve == "none" else [k for k in args.solve.split(",") if k]
     def run_solve(k):
