@@ -51,7 +51,26 @@ def global_matrix(c, dtype=np.float64, active=False, flux=None):
 CASES = [dict(d=2, n=2, L=4, T=2), dict(d=3, n=2, L=4, T=2), dict(d=3, n=3, L=3, T=2),
          dict(d=4, n=2, L=3, T=1), dict(d=1, n=3, L=5, T=2), dict(d=2, n=4, L=3, T=1),
          dict(d=2, n=2, L=5, T=1, refine=0.3), dict(d=3, n=2, L=4, T=2, refine=0.3),
-         dict(d=4, n=2, L=4, T=1, refine=0.2), dict(d=2, n=3, L=4, T=2, refine=0.4)]
+         dict(d=4, n=2, L=4, T=1, refine=0.2), dict(d=2, n=3, L=4, T=2, refine=0.4),
+         # n^d = 64 (one cell per wavefront), long connection lists at L=6
+         dict(d=3, n=4, L=6, T=1), dict(d=6, n=2, L=3, T=1)]
+
+
+_ORACLE = {}
+
+
+def _case_and_refs(spec):
+    """The case and its oracle results, computed once per spec."""
+    key = tuple(sorted(spec.items()))
+    if key not in _ORACLE:
+        c = grid_case(spec["d"], spec["n"], spec["L"], spec["T"], seed=3,
+                      refine=spec.get("refine", 0.0))
+        ref = gk.apply(c["d"], c["n"], c["table"], c["L"], c["T"], c["vA"], 1.5, c["x"], -0.5,
+                       c["y"])
+        ax = gk.apply(c["d"], c["n"], c["table"], c["L"], c["T"], c["vA"], 1.0, c["x"], 0.0,
+                      c["y"])
+        _ORACLE[key] = (c, ref, ax)
+    return _ORACLE[key]
 
 
 LAYOUTS = (1, 8, 2)  # lane per DOF, 8 lanes per DOF, wavefront per cell
@@ -60,10 +79,7 @@ LAYOUTS = (1, 8, 2)  # lane per DOF, 8 lanes per DOF, wavefront per cell
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("spec", CASES, ids=lambda s: "-".join(f"{k}{v}" for k, v in s.items()))
 def test_global_parity(spec, layout):
-    c = grid_case(spec["d"], spec["n"], spec["L"], spec["T"], seed=3,
-                  refine=spec.get("refine", 0.0))
-    ref = gk.apply(c["d"], c["n"], c["table"], c["L"], c["T"], c["vA"], 1.5, c["x"], -0.5,
-                   c["y"])
+    c, ref, ax = _case_and_refs(spec)
     m = global_matrix(c)
     m.set_tuning(waves_per_row=layout)
     y = dev(c["y"])
@@ -81,8 +97,7 @@ def test_global_parity(spec, layout):
     assert rel(host(y32), ref) < TOL32
     # implicit operator: x - dt*A x
     e = m.apply_euler(0.1, 1.0, dev(c["x"]), 0.0, dev(np.zeros_like(c["x"])))
-    eref = c["x"] - 0.1 * gk.apply(c["d"], c["n"], c["table"], c["L"], c["T"], c["vA"], 1.0,
-                                   c["x"], 0.0, c["y"])
+    eref = c["x"] - 0.1 * ax
     assert rel(host(e), eref) < TOL64
 
 
