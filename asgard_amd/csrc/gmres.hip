@@ -31,6 +31,8 @@ namespace akm
 {
 constexpr int gm_threads = 256;
 constexpr int gm_cols = 8; // Krylov columns per dot-product block
+// the Givens kernel stages 3 * restart values in LDS (64 KB)
+constexpr int gm_max_restart = 2000;
 
 template<typename P>
 struct gm_state
@@ -193,38 +195,65 @@ __global__ void __launch_bounds__(64)
 gm_givens(int nblk, const P *sq_partial, const P *h, int j, P *H, P *cs, P *sn, P *g, P tol,
           int rotate_always, gm_state<P> *st)
 {
+  // column j, then the previous rotations, staged in LDS so that the serial
+  // part below runs on registers and LDS (dynamic LDS: 3j + 1 values)
   __shared__ P sh[64];
+  extern __shared__ unsigned char gv_raw[];
+  P *lc = reinterpret_cast<P *>(gv_raw);
+  P *lcs = lc + (j + 1);
+  P *lsn = lcs + j;
+  for (int k = threadIdx.x; k <= j; k += 64)
+    lc[k] = h[k];
+  for (int k = threadIdx.x; k < j; k += 64)
+  {
+    lcs[k] = cs[k];
+    lsn[k] = sn[k];
+  }
   P s = 0;
   for (int b = threadIdx.x; b < nblk; b += 64)
     s += sq_partial[b];
-  s = block_sum(s, sh);
-  if (threadIdx.x != 0)
-    return;
-  const P nrm = sqrt(s);
-  st->nrm = nrm;
-  st->inv_nrm = (nrm > P{0}) ? P{1} / nrm : P{0};
+  s = block_sum(s, sh); // its barriers also order the LDS writes above
+  if (threadIdx.x == 0)
+  {
+    const P nrm = sqrt(s);
+    st->nrm = nrm;
+    st->inv_nrm = (nrm > P{0}) ? P{1} / nrm : P{0};
+    if (j < 0)
+    {
+      g[0] = nrm;
+      st->inner_res = nrm;
+    }
+    else
+    {
+      // apply the old rotations down the column, the running entry in a register
+      P x = lc[0];
+      for (int k = 0; k < j; k++)
+      {
+        P y = lc[k + 1];
+        dev_rot(x, y, lcs[k], lsn[k]);
+        lc[k] = x;
+        x = y;
+      }
+      P c, sv;
+      dev_rotg(x, nrm, c, sv);
+      lc[j] = x;
+      cs[j] = c;
+      sn[j] = sv;
+      const P res = fabs(sv * g[j]);
+      st->inner_res = res;
+      if (rotate_always != 0 || res > tol)
+      {
+        g[j + 1] = 0;
+        dev_rot(g[j], g[j + 1], c, sv);
+      }
+    }
+  }
   if (j < 0)
-  {
-    g[0] = nrm;
-    st->inner_res = nrm;
     return;
-  }
+  __syncthreads();
   P *col = H + (j * (j + 1)) / 2; // packed upper triangle, column j
-  for (int k = 0; k <= j; k++)
-    col[k] = h[k];
-  for (int k = 0; k < j; k++)
-    dev_rot(col[k], col[k + 1], cs[k], sn[k]);
-  P c, sv;
-  dev_rotg(col[j], nrm, c, sv);
-  cs[j] = c;
-  sn[j] = sv;
-  const P res = fabs(sv * g[j]);
-  st->inner_res = res;
-  if (rotate_always != 0 || res > tol)
-  {
-    g[j + 1] = 0;
-    dev_rot(g[j], g[j + 1], c, sv);
-  }
+  for (int k = threadIdx.x; k <= j; k += 64)
+    col[k] = lc[k];
 }
 
 template<typename P>
@@ -239,21 +268,40 @@ __global__ void __launch_bounds__(gm_threads) gm_scale(int64_t n, P *w, const gm
 }
 
 //! Packed upper-triangular back-solve H y = g (tpsv 'U','N','N'), in g.
+//! One workgroup; g (and H when it fits, `h_in_lds`) staged in LDS; column
+//! by column, the updates of one column spread over the threads (the same
+//! operations in the same order per entry as a serial solve).
 template<typename P>
-__global__ void gm_backsolve(int m, const P *H, P *g)
+__global__ void __launch_bounds__(256) gm_backsolve(int m, const P *H, P *g, int h_in_lds)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0)
-    return;
+  extern __shared__ unsigned char bs_raw[];
+  P *lg = reinterpret_cast<P *>(bs_raw);
+  P *lh = lg + m;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m; i += 256)
+    lg[i] = g[i];
+  if (h_in_lds)
+    for (int i = tid; i < m * (m + 1) / 2; i += 256)
+      lh[i] = H[i];
+  __syncthreads();
+  const P *Hs = h_in_lds ? lh : H;
   for (int j = m - 1; j >= 0; j--)
   {
-    const P hjj = H[j + (j * (j + 1)) / 2];
-    // a zero diagonal only follows an exact breakdown (residual already
-    // zero); that column then takes no part in the solution
-    g[j] = (hjj != P{0}) ? g[j] / hjj : P{0};
-    const P gj = g[j];
-    for (int i = 0; i < j; i++)
-      g[i] -= gj * H[i + (j * (j + 1)) / 2];
+    if (tid == 0)
+    {
+      const P hjj = Hs[j + (j * (j + 1)) / 2];
+      // a zero diagonal only follows an exact breakdown (residual already
+      // zero); that column then takes no part in the solution
+      lg[j] = (hjj != P{0}) ? lg[j] / hjj : P{0};
+    }
+    __syncthreads();
+    const P gj = lg[j];
+    for (int i = tid; i < j; i += 256)
+      lg[i] -= gj * Hs[i + (j * (j + 1)) / 2];
+    __syncthreads();
   }
+  for (int i = tid; i < m; i += 256)
+    g[i] = lg[i];
 }
 
 //! x += V[:, 0:m] y
@@ -464,7 +512,8 @@ struct gm_cycle
     gm_dots<P><<<grid, gm_threads, 0, s>>>(n, rpb, V, ldv, w, ncols, dotp);
     gm_reduce<P><<<ncols, gm_threads, 0, s>>>(nblk, dotp, h);
     gm_update<P><<<nblk, gm_threads, sizeof(P) * ncols, s>>>(n, rpb, V, ldv, w, ncols, h, sqp);
-    gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, j, H, cs, sn, g, tol, rotate_always, st);
+    gm_givens<P><<<1, 64, sizeof(P) * (3 * j + 1), s>>>(nblk, sqp, h, j, H, cs, sn, g, tol,
+                                                         rotate_always, st);
     gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, w, st);
     AKM_TRY(check_launch("gmres inner iteration"));
     AKM_HIP(hipMemcpyAsync(hst + (j & 1), st, sizeof(gm_state<P>), hipMemcpyDeviceToHost, s));
@@ -510,7 +559,9 @@ struct gm_cycle
   {
     if (m <= 0)
       return AKM_SUCCESS;
-    gm_backsolve<P><<<1, 1, 0, s>>>(m, H, g);
+    const size_t hbytes = sizeof(P) * (size_t(m) * (m + 1) / 2);
+    const int h_in_lds = (hbytes + sizeof(P) * m <= 48 * 1024) ? 1 : 0;
+    gm_backsolve<P><<<1, 256, sizeof(P) * m + (h_in_lds ? hbytes : 0), s>>>(m, H, g, h_in_lds);
     gm_xupdate<P><<<sblocks, gm_threads, sizeof(P) * m, s>>>(n, flexible ? Z : V, ldv, m, g, x);
     return check_launch("gmres update");
   }
@@ -749,6 +800,8 @@ extern "C" akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const 
   if (restart > n)
     return set_error(AKM_ERR_INVALID, "Number of inner iterations %d must be less than %lld!",
                      restart, static_cast<long long>(n));
+  if (restart > gm_max_restart)
+    return set_error(AKM_ERR_UNSUPPORTED, "gmres: restart %d above %d", restart, gm_max_restart);
   if (max_iter <= 0)
     max_iter = static_cast<int>(std::min<int64_t>(n, 1 << 30));
   AKM_TRY(ensure_workspace(*mat, restart));
@@ -812,6 +865,8 @@ extern "C" akm_status akm_gmres_euler_mixed(akm_matrix mat, akm_matrix mat32, ak
   if (restart > n)
     return set_error(AKM_ERR_INVALID, "Number of inner iterations %d must be less than %lld!",
                      restart, static_cast<long long>(n));
+  if (restart > gm_max_restart)
+    return set_error(AKM_ERR_UNSUPPORTED, "gmres: restart %d above %d", restart, gm_max_restart);
   if (max_cycles <= 0)
     max_cycles = static_cast<int>(std::min<int64_t>(n, 1 << 30));
   AKM_TRY(ensure_workspace(*mat, restart));

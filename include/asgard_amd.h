@@ -210,7 +210,9 @@ akm_status akm_destroy(akm_matrix mat);
  * restart/max_iter/tol <= 0 select the reference defaults
  * (default_gmres_restarts, n outer iterations, 1e-12 FP64 / 1e-6 FP32).
  * All vector work stays on the GPU.  err and iterations mirror gmres_info.
- * Returns AKM_ERR_INVALID if restart > n, as the reference throws.  On a
+ * Returns AKM_ERR_INVALID if restart > n, as the reference throws, and
+ * AKM_ERR_UNSUPPORTED if restart > 2000 (the reference's default is at most
+ * 200).  On a
  * global operator (akm_create_global_grid) the residual and every new basis
  * vector are divided by (1 - dt*pc), pc its Jacobi diagonal, as the
  * reference's global-mode solve does (src/solver.cpp:76-118); tol and err
