@@ -46,17 +46,18 @@ struct gm_state
 template<typename P>
 __device__ __forceinline__ P block_sum(P v, P *sh)
 {
-  // fixed-order block reduction (deterministic)
-  const int tid = threadIdx.x;
-  sh[tid] = v;
+  // fixed-order block reduction (deterministic): a shuffle tree inside each
+  // wavefront, then the wavefronts' sums in order; two barriers
+#pragma unroll
+  for (int off = wave_size / 2; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, wave_size);
+  const int wave = threadIdx.x / wave_size;
+  if (threadIdx.x % wave_size == 0)
+    sh[wave] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1)
-  {
-    if (tid < s)
-      sh[tid] += sh[tid + s];
-    __syncthreads();
-  }
-  P r = sh[0];
+  P r = 0;
+  for (int w = 0; w < static_cast<int>(blockDim.x) / wave_size; w++)
+    r += sh[w];
   __syncthreads();
   return r;
 }
