@@ -165,16 +165,17 @@ __global__ void __launch_bounds__(gk_threads)
 //! Cell layout: one wavefront per (chain, cell).  Lane (g, e), e < N, g <
 //! 64/N, takes every (64/N)-th connection of the cell's list starting at g;
 //! the lanes of one g read a connected cell's tensor together, and the list
-//! itself is loaded 64 entries at a time and passed between lanes.  Four
-//! connections are fetched before any is used (compile-time n = NT; NT = 0
-//! reads n at run time).  Returns this lane's partial sum for entry e (0 on
+//! itself is loaded 64 entries at a time and passed between lanes.  Two
+//! connections are fetched before either is used (compile-time n = NT; NT = 0
+//! reads n at run time; two keeps the kernel at 5+ wavefronts per SIMD, four
+//! measured slower).  Returns this lane's partial sum for entry e (0 on
 //! idle lanes).
 template<typename P, int NT>
 __device__ __forceinline__ P gk_cell_part(gk_args const &a, const P *__restrict__ vA, int t, int k,
                                           int fill, const P *__restrict__ in, bool in_active_only,
                                           int64_t cell, int lane)
 {
-  constexpr int U = 4;
+  constexpr int U = 2;
   const int N = a.N, d = a.d;
   const int n = (NT > 0) ? NT : a.n;
   const int G = wave_size / N;
