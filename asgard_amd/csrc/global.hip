@@ -270,6 +270,8 @@ __device__ __forceinline__ void gk_wave_sync()
 template<typename P>
 __device__ __forceinline__ P gk_cell_reduce(P v, P *red, int lane, int N)
 {
+  if (N == wave_size)
+    return v; // one lane group: nothing to add
   red[lane] = v;
   gk_wave_sync();
   P sum = 0;
