@@ -917,7 +917,11 @@ akm_status akm_global_jacobi(akm_matrix mat, double *pc)
     return set_error(AKM_ERR_INVALID, "global jacobi: not a global kron matrix");
   const int64_t count = mat->gk->active_cells * mat->N;
   if (count > 0 && mat->gk->pc.data() != nullptr)
-    AKM_HIP(hipMemcpy(pc, mat->gk->pc.data(), sizeof(double) * count, hipMemcpyDeviceToHost));
+  {
+    AKM_HIP(hipMemcpyAsync(pc, mat->gk->pc.data(), sizeof(double) * count, hipMemcpyDeviceToHost,
+                           mat->stream));
+    AKM_HIP(hipStreamSynchronize(mat->stream));
+  }
   else if (count > 0)
     std::fill(pc, pc + count, 0.0);
   return AKM_SUCCESS;
