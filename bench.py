@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
@@ -233,6 +234,53 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
     if with_cpu and w["cpu"]:
         out["cpu_baseline"] = solve_cpu_baseline(d, n, L, T, table, vA, dt, b.cpu().numpy(),
                                                  restart, x_plain.cpu().numpy())
+    return out
+
+
+def workload_variants(d, n, L, T, table, x, y64, stream, no_cpu):
+    """GFLOP/s of the bench grid with the repo's DG advection/mass operators
+    (parity vs the oracle unless no_cpu) and of the FP32 kronmult on the
+    random factors (its error vs the FP64 GPU result)."""
+    import torch
+
+    import asgard_amd
+    from asgard_amd import dg, grid
+
+    out = {}
+    xd = torch.from_numpy(x).cuda()
+    ops = dg.operators_advection(d, n, L)
+    vA_dg = grid.build_vA(d, n, T, L, ops)
+    mdg = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA_dg)
+    mdg.set_stream(stream.cuda_stream)
+    y = torch.zeros_like(xd)
+    mdg.autotune(xd, y)
+    ms = mdg.time_apply(1.0, xd, 0.0, y, 50)
+    mdg.apply(1.0, xd, 0.0, y)
+    rec = {"gflops": round(mdg.flops() / (ms * 1e-3) * 1e-9, 1),
+           "operators": "DG advection (upwind flux) in dimension t for term t, mass elsewhere"}
+    if not no_cpu:
+        import oracle
+
+        pntr, indx, iA = oracle.sparse_pattern(table, n=n, T=T, max_level=L)
+        ref = np.zeros_like(x)
+        oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA_dg, 1.0, x, 0.0, ref)
+        rec["parity_max_rel_err_vs_oracle"] = float(np.abs(y.cpu().numpy() - ref).max() /
+                                                    np.abs(ref).max())
+    out["dg_advection_mass"] = rec
+    _, vA, _ = make_workload(d, n, L, T)
+    m32 = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA.astype(np.float32),
+                                                dtype=np.float32)
+    m32.set_stream(stream.cuda_stream)
+    x32 = xd.float()
+    y32 = torch.zeros_like(x32)
+    m32.autotune(x32, y32)
+    ms32 = m32.time_apply(1.0, x32, 0.0, y32, 50)
+    m32.apply(1.0, x32, 0.0, y32)
+    err = float(((y32.double() - y64).abs().max() / y64.abs().max()).item())
+    out["fp32_random"] = {"gflops": round(m32.flops() / (ms32 * 1e-3) * 1e-9, 1),
+                          "max_rel_err_vs_fp64": err,
+                          "roofline_frac_fp32": round(m32.flops() / (ms32 * 1e-3) * 1e-12 /
+                                                      FP32_PEAK_TFLOPS, 4)}
     return out
 
 
@@ -504,6 +552,10 @@ def main():
         "cpu_baseline": cpu,
         "parity_max_rel_err_vs_oracle": parity,
     }
+    # the workload's other variants: the DG operators (advection in one
+    # dimension per term, mass elsewhere) and the FP32 kronmult
+    line["variants"] = workload_variants(d, n, L, T, table, x, yd, stream, args.no_cpu)
+
     # the same workload through the global operator (every factor active here)
     gmat = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA)
     gmat.set_stream(stream.cuda_stream)
