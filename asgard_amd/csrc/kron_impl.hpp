@@ -220,6 +220,127 @@ __device__ __forceinline__ void row_range(kron_args<P> const &a, int row, int64_
   }
 }
 
+// ------------------------------------------------- row-end reduction ---
+
+//! True when the column slots of a wavefront are the lane bits above the
+//! lane-group bits (G a power of two, registers in the untransposed
+//! layout), so that the slots can be summed by a tree of lane exchanges.
+//! G <= 8 keeps CPW >= 8 >= W, so the W * N partial sums fit in the
+//! CPW * (N + 1) reduction buffer.
+template<int n, int g>
+constexpr bool slot_tree_reduce()
+{
+  constexpr int G = ipow<n, g>::value;
+  return !transposed_exchange<n, g>() && (G & (G - 1)) == 0 && G <= 8;
+}
+
+//! One level of the slot tree.  Lanes `lane` and `lane ^ MASK` own the same
+//! accumulator entries [e, end) (stored from v[0]; slots at or past `end`
+//! are padding and hold zero).  The lower lane keeps the first ceil(LEN/2)
+//! slots, the upper lane the rest, each summed with the partner's copy;
+//! after the last level (MASK < STOP) every lane hands its real entries to
+//! `emit(entry, value)`.  The tree is fixed, so the sums are the same run to
+//! run.
+template<typename P, int LEN, int MASK, int STOP, typename F>
+__device__ __forceinline__ void slot_tree(const P (&v)[LEN], int lane, int e, int end, F &&emit)
+{
+  if constexpr (MASK < STOP)
+  {
+#pragma unroll
+    for (int i = 0; i < LEN; i++)
+      if (e + i < end)
+        emit(e + i, v[i]);
+  }
+  else
+  {
+    constexpr int H = (LEN + 1) / 2;
+    const bool upper = (lane & MASK) != 0;
+    P nv[H];
+#pragma unroll
+    for (int i = 0; i < H; i++)
+    {
+      const P lo = v[i];
+      const P hi = (H + i < LEN) ? v[(H + i < LEN) ? H + i : 0] : P(0);
+      const P recv = __shfl_xor(upper ? lo : hi, MASK, wave_size);
+      nv[i] = (upper ? hi : lo) + recv;
+    }
+    const int ne = upper ? e + H : e;
+    const int nend = upper ? end : (end < e + H ? end : e + H);
+    slot_tree<P, H, MASK / 2, STOP>(nv, lane, ne, nend, emit);
+  }
+}
+
+//! Sums the per-lane accumulators of a row over the column slots of every
+//! wavefront and stores the row.  With the slot tree each wavefront sums its
+//! slots in registers and writes N partial sums; otherwise the wavefronts
+//! take turns writing their slots to LDS.  The wavefronts' partial sums are
+//! then added in wavefront order.
+template<typename P, int D, int n, int g, int W>
+__device__ __forceinline__ void row_reduce_store(kron_args<P> const &a, int row,
+                                                 P (&acc)[ipow<n, D - g>::value], P *red)
+{
+  constexpr int N = ipow<n, D>::value;
+  constexpr int G = ipow<n, g>::value;
+  constexpr int R = N / G;
+  constexpr int CPW = wave_size / G;
+  constexpr int LDN = N + 1;
+  constexpr int nthreads = W * wave_size;
+  constexpr int EPT = (N + nthreads - 1) / nthreads;
+  constexpr bool TR = transposed_exchange<n, g>();
+  const int tid = threadIdx.x;
+  const int wave = tid / wave_size;
+  const int lane = tid - wave * wave_size;
+  const int slot = lane / G;
+  const int sub = lane - slot * G;
+  const int64_t ybase = int64_t(row) * N;
+  if constexpr (slot_tree_reduce<n, g>())
+  {
+    P *mine = red + wave * N + sub * R;
+    slot_tree<P, R, wave_size / 2, G>(acc, lane, 0, R, [&](int r, P v) { mine[r] = v; });
+    __syncthreads();
+    for (int e = tid; e < N; e += nthreads)
+    {
+      P tot = red[e];
+      for (int wv = 1; wv < W; wv++)
+        tot += red[wv * N + e];
+      store_row(a, ybase + e, tot);
+    }
+  }
+  else
+  {
+    P tot[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++)
+      tot[i] = 0;
+    for (int wv = 0; wv < W; wv++)
+    {
+      if (wave == wv && slot < CPW)
+      {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          red[slot * LDN + acc_index<N, G, R, TR>(sub, r)] = acc[r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < EPT; i++)
+      {
+        const int e = tid + i * nthreads;
+        if (e < N)
+          for (int sl = 0; sl < CPW; sl++)
+            tot[i] += red[sl * LDN + e];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < EPT; i++)
+    {
+      const int e = tid + i * nthreads;
+      if (e < N)
+        store_row(a, ybase + e, tot[i]);
+    }
+  }
+}
+
 // ---------------------------------------------------- register kernel ---
 
 #ifndef AKM_WAVES_PER_EU
@@ -235,9 +356,7 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
   constexpr int G = ipow<n, g>::value;
   constexpr int R = N / G;
   constexpr int CPW = wave_size / G; // columns per wavefront step
-  constexpr int LDN = N + 1;         // padded LDS row
   constexpr int nn = n * n;
-  constexpr int EPT = (N + W * wave_size - 1) / (W * wave_size);
   static_assert(G <= wave_size, "lane group larger than a wavefront");
 
   constexpr bool TR = transposed_exchange<n, g>();
@@ -272,24 +391,98 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
   }
 
   const int64_t tstride = int64_t(D) * nn; // term stride inside a grid block
-  for (int64_t j0 = jbeg + int64_t(wave) * CPW; j0 < jend; j0 += int64_t(W) * CPW)
+  if constexpr (MODE == mode_grid)
   {
-    const int64_t j = j0 + slot;
-    if (lane_used && j < jend)
+    // Software pipeline over the wavefront's column steps: the column id two
+    // steps ahead and the next column's 1D indices are fetched at the start
+    // of a step, its connect_1d positions and x slice under the last term.
+    const int64_t blk = tstride * a.T; // one connection's factor blocks
+    const int64_t jstep = int64_t(W) * CPW;
+    int64_t j = jbeg + int64_t(wave) * CPW + slot;
+    bool valid = lane_used && j < jend && a.T > 0;
+    P X[R];
+    int64_t cbase[D];
+    if (valid)
     {
-      const int col = (MODE == mode_dense) ? int(j) : a.indx[j];
-      const P *__restrict__ xc = a.x + int64_t(col) * N + sub * R;
-      int64_t cbase[D];
-      if constexpr (MODE == mode_grid)
+      const int col = a.indx[j];
+#pragma unroll
+      for (int k = 0; k < D; k++)
+        cbase[k] = int64_t(a.off1d[r1d[k] + a.elem[int64_t(col) * D + k]]) * blk + k * nn;
+      const P *xc = a.x + int64_t(col) * N + sub * R;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        X[r] = xc[r];
+    }
+    int col_n = 0;
+    if (valid && j + jstep < jend)
+      col_n = a.indx[j + jstep];
+    for (int64_t j0 = jbeg + int64_t(wave) * CPW; j0 < jend; j0 += jstep)
+    {
+      const int64_t jn = j + jstep;
+      const bool valid_next = valid && jn < jend;
+      int en[D];
+      if (valid_next)
       {
 #pragma unroll
         for (int k = 0; k < D; k++)
-        {
-          const int off = a.off1d[r1d[k] + a.elem[int64_t(col) * D + k]];
-          cbase[k] = int64_t(off) * (tstride * a.T) + k * nn;
-        }
+          en[k] = a.elem[int64_t(col_n) * D + k];
       }
-      else if constexpr (MODE == mode_dense)
+      int col_nn = 0;
+      if (valid_next && jn + jstep < jend)
+        col_nn = a.indx[jn + jstep];
+      if (valid)
+      {
+        const P *Ak[D];
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          Ak[k] = a.vA + cbase[k];
+        P w[R];
+        for (int t = 0; t < a.T - 1; t++)
+        {
+          if constexpr (TR)
+            contract_term_t<P, D, n, g, R, false>(X, w, acc, Ak, sub, tb);
+          else
+            contract_term<P, D, n, g, R, false>(X, w, acc, Ak, lane, sub);
+#pragma unroll
+          for (int k = 0; k < D; k++)
+            Ak[k] += tstride;
+        }
+        // last term: contract from a copy in w while the next column's
+        // factor offsets and x slice arrive
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          w[r] = X[r];
+        if (valid_next)
+        {
+#pragma unroll
+          for (int k = 0; k < D; k++)
+            cbase[k] = int64_t(a.off1d[r1d[k] + en[k]]) * blk + k * nn;
+          const P *xn = a.x + int64_t(col_n) * N + sub * R;
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            X[r] = xn[r];
+        }
+        if constexpr (TR)
+          contract_term_t<P, D, n, g, R, true>(X, w, acc, Ak, sub, tb);
+        else
+          contract_term<P, D, n, g, R, true>(X, w, acc, Ak, lane, sub);
+      }
+      j = jn;
+      valid = valid_next;
+      col_n = col_nn;
+    }
+  }
+  else
+  {
+    for (int64_t j0 = jbeg + int64_t(wave) * CPW; j0 < jend; j0 += int64_t(W) * CPW)
+    {
+      const int64_t j = j0 + slot;
+      if (!(lane_used && j < jend))
+        continue;
+      const int col = (MODE == mode_dense) ? int(j) : a.indx[j];
+      const P *__restrict__ xc = a.x + int64_t(col) * N + sub * R;
+      int64_t cbase[D];
+      if constexpr (MODE == mode_dense)
       {
         const int64_t vstride = int64_t(a.nb) * a.nb * nn;
 #pragma unroll
@@ -306,13 +499,7 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
       for (int t = 0; t < a.T; t++)
       {
         const P *Ak[D];
-        if constexpr (MODE == mode_grid)
-        {
-#pragma unroll
-          for (int k = 0; k < D; k++)
-            Ak[k] = a.vA + cbase[k] + t * tstride;
-        }
-        else if constexpr (MODE == mode_dense)
+        if constexpr (MODE == mode_dense)
         {
           const P *tp = a.terms[t];
 #pragma unroll
@@ -336,39 +523,7 @@ __device__ __forceinline__ void kron_row_body(kron_args<P> const &a, int row, P 
   }
   if constexpr (TR)
     __syncthreads(); // red may share LDS with the transpose slices
-
-  // sum the column slots of every wavefront, in a fixed order
-  P tot[EPT];
-#pragma unroll
-  for (int i = 0; i < EPT; i++)
-    tot[i] = 0;
-  for (int wv = 0; wv < W; wv++)
-  {
-    if (wave == wv && lane_used)
-    {
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        red[slot * LDN + acc_index<N, G, R, TR>(sub, r)] = acc[r];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < EPT; i++)
-    {
-      const int e = tid + i * W * wave_size;
-      if (e < N)
-        for (int sl = 0; sl < CPW; sl++)
-          tot[i] += red[sl * LDN + e];
-    }
-    __syncthreads();
-  }
-  const int64_t ybase = int64_t(row) * N;
-#pragma unroll
-  for (int i = 0; i < EPT; i++)
-  {
-    const int e = tid + i * W * wave_size;
-    if (e < N)
-      store_row(a, ybase + e, tot[i]);
-  }
+  row_reduce_store<P, D, n, g, W>(a, row, acc, red);
 }
 
 template<typename P, int D, int n, int g, int MODE, int W>
@@ -488,9 +643,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
   constexpr int G = ipow<n, g>::value;
   constexpr int R = N / G;
   constexpr int CPW = wave_size / G;
-  constexpr int LDN = N + 1;
   constexpr int nnp = padded_block<P>(n);
-  constexpr int EPT = (N + W * wave_size - 1) / (W * wave_size);
   constexpr int nthreads = W * wave_size;
   static_assert(G <= wave_size, "lane group larger than a wavefront");
 
@@ -624,38 +777,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_grid_lds(kron_args<P> a)
     valid = valid_next;
   }
   __syncthreads(); // factor tables are dead from here on
-
-  P tot[EPT];
-#pragma unroll
-  for (int i = 0; i < EPT; i++)
-    tot[i] = 0;
-  for (int wv = 0; wv < W; wv++)
-  {
-    if (wave == wv && lane_used)
-    {
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        red[slot * LDN + acc_index<N, G, R, TR>(sub, r)] = acc[r];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < EPT; i++)
-    {
-      const int e = tid + i * nthreads;
-      if (e < N)
-        for (int sl = 0; sl < CPW; sl++)
-          tot[i] += red[sl * LDN + e];
-    }
-    __syncthreads();
-  }
-  const int64_t ybase = int64_t(row) * N;
-#pragma unroll
-  for (int i = 0; i < EPT; i++)
-  {
-    const int e = tid + i * nthreads;
-    if (e < N)
-      store_row(a, ybase + e, tot[i]);
-  }
+  row_reduce_store<P, D, n, g, W>(a, row, acc, red);
 }
 
 // ------------------------------------------------------- generic kernel ---
