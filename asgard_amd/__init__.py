@@ -46,6 +46,7 @@ EXPORTS = (
     "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
     "akm_create_sparse_grid", "akm_create_global_grid", "akm_global_info",
     "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
+    "akm_update_operators",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
     "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
@@ -117,6 +118,7 @@ def load_library(path: str = LIB_PATH):
     L.akm_create_dense.argtypes = [_i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i,
                                    ctypes.POINTER(_vp)]
     L.akm_update_coefficients.argtypes = [_vp, _vp]
+    L.akm_update_operators.argtypes = [_vp, _vp, ctypes.c_int64]
     L.akm_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i]
     L.akm_apply_euler.argtypes = [_vp, _d, _d, _vp, _d, _vp]
     L.akm_matrix_info.argtypes = [_vp] + [ctypes.POINTER(_i64)] * 5
@@ -422,6 +424,31 @@ class KronmultMatrix:
         else:
             arr = _np(values, self.dtype)
             _check(load_library().akm_update_coefficients(self._h, arr.ctypes.data))
+
+    def update_operators(self, ops):
+        """update_kronmult_coefficients on the GPU: rebuilds the stored
+        factor blocks from num_terms * d square device tensors, ops[t*d + k]
+        the operator of term t in dimension k.  Each tensor holds its
+        operator's transpose in row-major order (i.e. the operator in
+        column-major order, as ``torch.as_tensor(A.T)`` of a matrix A)."""
+        if len(ops) != self.num_terms * self.num_dimensions:
+            raise AkmInvalid(AKM_ERR_INVALID, "update_operators needs num_terms * d operators")
+        import torch
+        want = torch.float64 if self.dtype == np.float64 else torch.float32
+        lda = None
+        for op in ops:
+            if not (_is_torch(op) and op.is_cuda):
+                raise AkmInvalid(AKM_ERR_INVALID, "update_operators needs device tensors")
+            if op.dtype != want or op.dim() != 2 or op.shape[0] != op.shape[1] \
+                    or not op.is_contiguous():
+                raise AkmInvalid(AKM_ERR_INVALID,
+                                 "update_operators needs contiguous square tensors of the "
+                                 "handle's precision")
+            if lda is not None and op.shape[0] != lda:
+                raise AkmInvalid(AKM_ERR_INVALID, "update_operators needs one operator size")
+            lda = op.shape[0]
+        ptrs = (ctypes.c_void_p * len(ops))(*[op.data_ptr() for op in ops])
+        _check(load_library().akm_update_operators(self._h, ptrs, int(lda or 0)))
 
 
 def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, tol=-1.0):

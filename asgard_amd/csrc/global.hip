@@ -385,6 +385,7 @@ struct akm_global_s
   akm::dev_buffer<unsigned char> work;
   akm::dev_buffer<double> pc; // Jacobi diagonal (active DOFs)
   std::vector<int> c1_pntr_h, c1_indx_h, c1_diag_h;
+  akm::dev_buffer<int> c1_diag; // device copy, made by the first device update
   int64_t flops = 0;
 };
 
@@ -535,6 +536,56 @@ int global_update(akm_matrix_s &m, const void *vA)
   AKM_TRY(m.gk->pc.upload(pc.data(), int64_t(pc.size()), m.stream));
   AKM_HIP(hipStreamSynchronize(m.stream));
   return AKM_SUCCESS;
+}
+
+//! jacobi_diagonal on the device, one thread per active DOF, with the same
+//! operations in the same order (no contraction into fma), so the two agree
+//! bit for bit.
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gk_jacobi(int64_t active_cells, int d, int n, int T, int64_t N, const int *elem,
+              const int *c1_diag, const P *vA, double *pc)
+{
+  const int64_t nn = int64_t(n) * n;
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < active_cells * N;
+       idx += int64_t(gridDim.x) * blockDim.x)
+  {
+    const int64_t c = idx / N, e = idx - c * N;
+    double sum = 0;
+    for (int t = 0; t < T; t++)
+    {
+      double prod = 1;
+      int64_t rest = e;
+      for (int k = d - 1; k >= 0; k--)
+      {
+        const int s = static_cast<int>(rest % n);
+        rest /= n;
+        const int64_t q = c1_diag[elem[c * d + k]];
+        prod = __dmul_rn(prod, static_cast<double>(vA[((q * T + t) * d + k) * nn + s * n + s]));
+      }
+      sum = __dadd_rn(sum, prod);
+    }
+    pc[idx] = sum;
+  }
+}
+
+int global_device_jacobi(akm_matrix_s &m)
+{
+  akm_global_s &g = *m.gk;
+  if (g.c1_diag.size() == 0)
+    AKM_TRY(g.c1_diag.upload(g.c1_diag_h.data(), int64_t(g.c1_diag_h.size()), m.stream));
+  const int64_t work = g.active_cells * m.N;
+  if (work == 0)
+    return AKM_SUCCESS;
+  if (m.prec == AKM_F64)
+    gk_jacobi<double><<<gk_blocks(work), gk_threads, 0, m.stream>>>(
+        g.active_cells, m.d, m.n, m.T, m.N, g.elem.data(), g.c1_diag.data(),
+        reinterpret_cast<const double *>(m.vA.data()), g.pc.data());
+  else
+    gk_jacobi<float><<<gk_blocks(work), gk_threads, 0, m.stream>>>(
+        g.active_cells, m.d, m.n, m.T, m.N, g.elem.data(), g.c1_diag.data(),
+        reinterpret_cast<const float *>(m.vA.data()), g.pc.data());
+  return check_launch("global kron Jacobi diagonal");
 }
 
 int64_t global_flops(akm_matrix_s const &m) { return m.gk ? m.gk->flops : 0; }

@@ -51,6 +51,11 @@ struct akm_matrix_s
   // workspaces for host-resident vectors
   akm::dev_buffer<unsigned char> xws, yws;
 
+  // operator update on the device: connect_1d (rows, then columns) in vA
+  // order, and the device copy of the caller's operator pointers
+  akm::dev_buffer<int> op_conn;
+  akm::dev_buffer<const void *> op_ptrs;
+
   hipStream_t stream = nullptr;
 
   std::unique_ptr<akm_gmres_workspace, void (*)(akm_gmres_workspace *)> gmres{nullptr,
@@ -79,6 +84,9 @@ int global_apply(akm_matrix_s &m, double alpha, const void *x, double beta, void
                  double gamma, const void *xg, bool jacobi, double jdt);
 //! Global operator: new factor values (same vA layout), Jacobi diagonal rebuilt.
 int global_update(akm_matrix_s &m, const void *vA);
+//! Global operator: Jacobi diagonal rebuilt on the device from the stored
+//! factor blocks (after they were written on the device).
+int global_device_jacobi(akm_matrix_s &m);
 //! Global operator: flops per apply as the reference counts them.
 int64_t global_flops(akm_matrix_s const &m);
 //! Global operator: lanes per DOF row in use (1 or 8).

@@ -154,6 +154,17 @@ akm_status akm_create_dense(int d, int n, int rows, int cols, int num_terms,
  * const void *const * of num_terms arrays) keeping the index lists. */
 akm_status akm_update_coefficients(akm_matrix mat, const void *values);
 
+/* update_kronmult_coefficients (src/asgard_kronmult_matrix.cpp:710-811) on
+ * the GPU: the stored factor blocks are gathered on the device from the
+ * dense 1D operator matrices, with no host copy.  ops is a host array of
+ * num_terms * d device pointers; ops[t*d + k] is the lda x lda column-major
+ * operator of term t in dimension k (in the handle's precision), lda >=
+ * n * 2^max_level for sparse-grid and global handles, >= n * num_1d_blocks
+ * for dense ones.  Global handles also get their Jacobi diagonal rebuilt on
+ * the device.  Handles made from an explicit iA list (akm_create_sparse)
+ * return AKM_ERR_UNSUPPORTED. */
+akm_status akm_update_operators(akm_matrix mat, const void *const *ops, int64_t lda);
+
 /* kronmult_matrix<P>::apply<resource>(alpha, x, beta, y)
  * (src/asgard_kronmult_matrix.hpp:340-478): y = alpha*A*x + beta*y.
  * With no terms: y = beta*y.  beta == 0 never reads y. */
