@@ -46,7 +46,7 @@ EXPORTS = (
     "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
     "akm_create_sparse_grid", "akm_create_global_grid", "akm_global_info",
     "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
-    "akm_update_operators",
+    "akm_update_operators", "akm_imex_step",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
     "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
@@ -56,6 +56,10 @@ EXPORTS = (
 )
 
 GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
+
+#: akm_imex_hook and the events it runs at
+IMEX_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
+IMEX_EXPLICIT_1, IMEX_IMPLICIT_1, IMEX_EXPLICIT_2, IMEX_IMPLICIT_2 = 0, 1, 2, 3
 
 
 class SolveStats(ctypes.Structure):
@@ -119,6 +123,8 @@ def load_library(path: str = LIB_PATH):
                                    ctypes.POINTER(_vp)]
     L.akm_update_coefficients.argtypes = [_vp, _vp]
     L.akm_update_operators.argtypes = [_vp, _vp, ctypes.c_int64]
+    L.akm_imex_step.argtypes = [_vp, _vp, _d, _vp, _i, _i, _d, _i, IMEX_HOOK, _vp,
+                                ctypes.POINTER(_d), ctypes.POINTER(_i)]
     L.akm_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i]
     L.akm_apply_euler.argtypes = [_vp, _d, _d, _vp, _d, _vp]
     L.akm_matrix_info.argtypes = [_vp] + [ctypes.POINTER(_i64)] * 5
@@ -449,6 +455,41 @@ class KronmultMatrix:
             lda = op.shape[0]
         ptrs = (ctypes.c_void_p * len(ops))(*[op.data_ptr() for op in ops])
         _check(load_library().akm_update_operators(self._h, ptrs, int(lda or 0)))
+
+
+def imex_step(explicit: KronmultMatrix, implicit: KronmultMatrix, dt, f, restart=-1,
+              max_iter=-1, tol=-1.0, collision=True, hook=None):
+    """imex_advance with device vectors: one SSP-RK2 IMEX step on the FP64
+    CUDA tensor f, in place.  hook(event, f_ptr) runs before each explicit
+    apply and each implicit solve (events IMEX_EXPLICIT_1 .. IMEX_IMPLICIT_2)
+    and may update either operator.  Returns (GmresInfo of each solve,
+    converged)."""
+    pf, on_dev = _vec_ptr(f, np.float64)
+    if not on_dev:
+        raise AkmInvalid(AKM_ERR_INVALID, "imex_step needs a device vector")
+    failure = []
+
+    def trampoline(_user, event, ptr):
+        try:
+            hook(event, ptr)
+            return 0
+        except BaseException as e:  # reported after the step returns
+            failure.append(e)
+            return 1
+
+    cb = IMEX_HOOK(trampoline) if hook is not None else IMEX_HOOK()
+    errs = (_d * 2)()
+    its = (_i * 2)()
+    try:
+        rc = _check(load_library().akm_imex_step(explicit._h, implicit._h, float(dt), pf,
+                                                 int(restart), int(max_iter), float(tol),
+                                                 int(bool(collision)), cb, None, errs, its),
+                    allow_not_converged=True)
+    except AkmError:
+        if failure:
+            raise failure[0]
+        raise
+    return [GmresInfo(errs[i], its[i]) for i in range(2)], rc == AKM_SUCCESS
 
 
 def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, tol=-1.0):

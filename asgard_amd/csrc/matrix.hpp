@@ -55,6 +55,8 @@ struct akm_matrix_s
   // order, and the device copy of the caller's operator pointers
   akm::dev_buffer<int> op_conn;
   akm::dev_buffer<const void *> op_ptrs;
+  // IMEX step vectors (four of output_size), on the implicit handle
+  akm::dev_buffer<double> imex_ws;
 
   hipStream_t stream = nullptr;
 

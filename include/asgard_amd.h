@@ -232,6 +232,35 @@ akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, in
                            int max_iter, double tol, int on_device, double *err,
                            int *iterations);
 
+/* ----------------------------------------------------------------- IMEX -- */
+
+/* Events of an IMEX step at which the caller's hook runs. */
+#define AKM_IMEX_EXPLICIT_1 0 /* before E f0; f = f0 */
+#define AKM_IMEX_IMPLICIT_1 1 /* before the stage-1 solve; f = its right-hand side */
+#define AKM_IMEX_EXPLICIT_2 2 /* before E f1; f = the stage-1 result */
+#define AKM_IMEX_IMPLICIT_2 3 /* before the stage-2 solve; f = its right-hand side */
+
+/* Caller code run inside an IMEX step with the device vector of the event
+ * (the stream is synchronized first); returns 0 to continue. */
+typedef int (*akm_imex_hook)(void *user, int event, const void *f);
+
+/* imex_advance (src/time_advance.cpp:760-1000) with imex_resrc =
+ * resource::device: one SSP-RK2 IMEX step on the FP64 device vector f,
+ *   f1s = f0 + dt E f0;  solve (I - dt B) f1 = f1s (initial guess f1s);
+ *   f2s = (f0 + f1 + dt E f1) / 2;  solve (I - dt/2 B) f = f2s (guess f1),
+ * E = explicit_op, B = implicit_op (square, same size).  Without
+ * `collision` the two solves are skipped (f1 = f1s, f = f2s), as in the
+ * reference's non-collision branch.  `hook` (may be NULL) runs at the four
+ * events above, where the reference recomputes moments and regenerates and
+ * resets coefficients (calculate_moments, generate_all_coefficients,
+ * reset_coefficients); it may update either handle (akm_update_operators).
+ * restart, max_iter and tol are those of akm_gmres_euler; errors[2] and
+ * iterations[2] (may be NULL) receive each solve's gmres_info.  Returns
+ * AKM_ERR_NOT_CONVERGED when a solve stopped above the tolerance. */
+akm_status akm_imex_step(akm_matrix explicit_op, akm_matrix implicit_op, double dt, void *f,
+                         int restart, int max_iter, double tol, int collision,
+                         akm_imex_hook hook, void *user, double *errors, int *iterations);
+
 /* A right preconditioner of the flexible solver (see akm_mlp_create). */
 typedef struct akm_precond_s *akm_precond;
 
