@@ -226,6 +226,19 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
     tinfo["seconds"] = round(t_train, 3)
     out["training"] = tinfo
     mixed(m32, pre, "fp32_refinement_learned")
+    # the same network on the global operator (one operator: on a standard
+    # grid the global and pair operators agree for any factors)
+    for rep in range(2):
+        xg = b.clone()
+        gst, tg = timed(lambda: asgard_amd.gmres_euler_mixed(dt, g64, xg, b, mat32=g32,
+                                                             precond=pre, restart=restart,
+                                                             tol=1e-10))
+    rec = gst.as_dict()
+    rec.update(seconds=round(tg, 4), speedup_vs_plain=round(plain["seconds"] / tg, 3),
+               speedup_vs_global_plain=round(out["global_plain_fp64_jacobi"]["seconds"] / tg,
+                                             3),
+               rel_residual_check=true_rel(xg))
+    out["global_fp32_refinement_learned"] = rec
     mixed(None, None, "fp64_flexible")
     mixed(None, pre, "fp64_flexible_learned")
     if w["rhs"] == "operator":
