@@ -46,7 +46,7 @@ EXPORTS = (
     "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
     "akm_create_sparse_grid", "akm_create_global_grid", "akm_global_info",
     "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
-    "akm_update_operators", "akm_imex_step",
+    "akm_update_operators", "akm_imex_step", "akm_dg_term",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
     "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
@@ -56,6 +56,14 @@ EXPORTS = (
 )
 
 GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
+
+class DgPartial(ctypes.Structure):
+    """akm_dg_partial: one partial term of a 1D DG operator."""
+    _fields_ = [("kind", ctypes.c_int), ("flux_scale", ctypes.c_double),
+                ("gq", ctypes.c_void_p), ("ge", ctypes.c_void_p)]
+
+
+DG_MASS, DG_DIV, DG_GRAD = 0, 1, 2
 
 #: akm_imex_hook and the events it runs at
 IMEX_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
@@ -123,6 +131,7 @@ def load_library(path: str = LIB_PATH):
                                    ctypes.POINTER(_vp)]
     L.akm_update_coefficients.argtypes = [_vp, _vp]
     L.akm_update_operators.argtypes = [_vp, _vp, ctypes.c_int64]
+    L.akm_dg_term.argtypes = [_i, _i, _d, _d, _i, ctypes.POINTER(DgPartial), _i, _vp, _vp, _vp]
     L.akm_imex_step.argtypes = [_vp, _vp, _d, _vp, _i, _i, _d, _i, IMEX_HOOK, _vp,
                                 ctypes.POINTER(_d), ctypes.POINTER(_i)]
     L.akm_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i]
@@ -490,6 +499,53 @@ def imex_step(explicit: KronmultMatrix, implicit: KronmultMatrix, dt, f, restart
             raise failure[0]
         raise
     return [GmresInfo(errs[i], its[i]) for i in range(2)], rc == AKM_SUCCESS
+
+
+def dg_quadrature(n, level, domain=(-1.0, 1.0), nq=None):
+    """Points where akm_dg_term takes the coefficients: the nq Gauss-Legendre
+    points of every cell ((2^level, nq), ascending within a cell) and the
+    2^level + 1 cell edges.  nq defaults to max(10, n + 1)."""
+    a, b = domain
+    cells = 1 << level
+    h = (b - a) / cells
+    nq = nq or max(10, n + 1)
+    x, _ = np.polynomial.legendre.leggauss(nq)
+    pts = a + h * (np.arange(cells)[:, None] + 0.5 * x[None, :] + 0.5)
+    return pts, a + h * np.arange(cells + 1), nq
+
+
+def dg_term(n, level, partials, F, domain=(-1.0, 1.0), nq=None, stream=None):
+    """generate_coefficients on the GPU for one 1D term (periodic domain).
+
+    partials: list of (kind, flux_scale, gq, ge) with kind DG_MASS / DG_DIV /
+    DG_GRAD and gq, ge FP64 CUDA tensors of the coefficient at the points of
+    :func:`dg_quadrature` (ge may be None for mass terms).  F: the multiwavelet
+    transform as a CUDA tensor holding it column-major (``torch.as_tensor(F.T)``).
+    Returns the operator as an N x N CUDA tensor holding it column-major, the
+    form :meth:`KronmultMatrix.update_operators` takes."""
+    import torch
+    N = n << level
+    nq = nq or max(10, n + 1)
+    parts = (DgPartial * len(partials))()
+    keep = []
+    for i, (kind, flux, gq, ge) in enumerate(partials):
+        gq = gq.contiguous()
+        keep.append(gq)
+        ptr_e = None
+        if ge is not None:
+            ge = ge.contiguous()
+            keep.append(ge)
+            ptr_e = ge.data_ptr()
+        parts[i] = DgPartial(int(kind), float(flux), gq.data_ptr(), ptr_e)
+    Fc = F.contiguous()
+    if Fc.shape != (N, N) or Fc.dtype != torch.float64 or not Fc.is_cuda:
+        raise AkmInvalid(AKM_ERR_INVALID, "dg_term needs an N x N FP64 CUDA transform")
+    out = torch.empty((N, N), dtype=torch.float64, device=Fc.device)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(load_library().akm_dg_term(int(n), int(level), float(domain[0]), float(domain[1]),
+                                      int(nq), parts, len(partials), Fc.data_ptr(),
+                                      out.data_ptr(), s))
+    return out
 
 
 def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, tol=-1.0):
