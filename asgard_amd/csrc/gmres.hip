@@ -4,10 +4,11 @@
 // akm_gmres_euler is the algorithm of solver::simple_gmres
 // (src/solver.cpp:156-298) with every vector and Hessenberg operation on the
 // GPU.  Per inner iteration: one fused apply (kronmult + identity), the
-// classical Gram-Schmidt dot products V^T w, the update w -= V h fused with
-// the partial sums of |w|^2, one single-wave kernel that finishes the norm,
+// classical Gram-Schmidt dot products V^T w, the update w -= V h (which first
+// reduces the dot partials to h) fused with the partial sums of |w|^2, and
+// one kernel that finishes the norm and scales w while its first workgroup
 // rotates the new Hessenberg column and the right-hand side (reference BLAS
-// rot/rotg semantics) and writes the residual estimate, and the scaling of w.
+// rot/rotg semantics) and writes the residual estimate.
 // Reductions use fixed block partitions and orders, so results repeat
 // bit for bit.  The host reads one scalar per iteration to test convergence.
 //
@@ -62,6 +63,16 @@ __device__ __forceinline__ P block_sum(P v, P *sh)
   return r;
 }
 
+//! Fixed-order sum over one wavefront (the shuffle tree of block_sum).
+template<typename P>
+__device__ __forceinline__ P wave_sum(P v)
+{
+#pragma unroll
+  for (int off = wave_size / 2; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, wave_size);
+  return v;
+}
+
 //! partial[k*nblk + bx] = sum over rows of block bx of V_k[i] * w[i],
 //! for k in [k0, k0 + gm_cols) intersect [0, ncols).
 template<typename P>
@@ -94,30 +105,34 @@ gm_dots(int64_t n, int64_t rows_per_block, const P *V, int64_t ldv, const P *w, 
   }
 }
 
-//! h[k] = sum_b partial[k*nblk + b] (one block per k).
-template<typename P>
-__global__ void __launch_bounds__(gm_threads) gm_reduce(int nblk, const P *partial, P *h)
-{
-  __shared__ P sh[gm_threads];
-  P s = 0;
-  for (int b = threadIdx.x; b < nblk; b += gm_threads)
-    s += partial[int64_t(blockIdx.x) * nblk + b];
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0)
-    h[blockIdx.x] = s;
-}
-
-//! w -= V[:, 0:ncols] h; sq_partial[bx] = sum of w^2 over the block's rows.
+//! w -= V[:, 0:ncols] h and sq_partial[bx] = sum of w^2 over the block's
+//! rows, with h formed first by every block from the dot partials:
+//! h[k] = sum_b partial[k*nblk + b] (one wavefront per column, lanes striding
+//! over the blocks, then the shuffle tree; the same order in every block).
+//! Block 0 stores h.
 template<typename P>
 __global__ void __launch_bounds__(gm_threads)
-gm_update(int64_t n, int64_t rows_per_block, const P *V, int64_t ldv, P *w, int ncols,
-          const P *h, P *sq_partial)
+gm_reduce_update(int64_t n, int64_t rows_per_block, const P *V, int64_t ldv, P *w, int ncols,
+                 const P *partial, P *h, P *sq_partial)
 {
   __shared__ P sh[gm_threads];
   extern __shared__ unsigned char hs_raw[];
   P *hs = reinterpret_cast<P *>(hs_raw);
-  for (int k = threadIdx.x; k < ncols; k += gm_threads)
-    hs[k] = h[k];
+  const int nblk = gridDim.x;
+  const int wave = threadIdx.x / wave_size, lane = threadIdx.x % wave_size;
+  for (int k = wave; k < ncols; k += gm_threads / wave_size)
+  {
+    P v = 0;
+    for (int b = lane; b < nblk; b += wave_size)
+      v += partial[int64_t(k) * nblk + b];
+    v = wave_sum(v);
+    if (lane == 0)
+    {
+      hs[k] = v;
+      if (blockIdx.x == 0)
+        h[k] = v;
+    }
+  }
   __syncthreads();
   const int64_t r0 = blockIdx.x * rows_per_block;
   const int64_t r1 = min(n, r0 + rows_per_block);
@@ -255,6 +270,82 @@ gm_givens(int nblk, const P *sq_partial, const P *h, int j, P *H, P *cs, P *sn, 
   P *col = H + (j * (j + 1)) / 2; // packed upper triangle, column j
   for (int k = threadIdx.x; k <= j; k += 64)
     col[k] = lc[k];
+}
+
+//! gm_givens for column j >= 0 and gm_scale in one launch: every block
+//! finishes the norm from the partials (the first wavefront, in gm_givens'
+//! order) and scales its rows of w; block 0 also rotates the Hessenberg
+//! column and the right-hand side and writes the state (dynamic LDS:
+//! 3j + 1 values).
+template<typename P>
+__global__ void __launch_bounds__(gm_threads)
+gm_givens_scale(int64_t n, P *w, int nblk, const P *sq_partial, const P *h, int j, P *H, P *cs,
+                P *sn, P *g, P tol, int rotate_always, gm_state<P> *st)
+{
+  __shared__ P s_nrm;
+  extern __shared__ unsigned char gs_raw[];
+  if (threadIdx.x < wave_size)
+  {
+    P s = 0;
+    for (int b = threadIdx.x; b < nblk; b += wave_size)
+      s += sq_partial[b];
+    s = wave_sum(s);
+    if (threadIdx.x == 0)
+      s_nrm = sqrt(s);
+  }
+  P *lc = reinterpret_cast<P *>(gs_raw);
+  P *lcs = lc + (j + 1);
+  P *lsn = lcs + j;
+  if (blockIdx.x == 0)
+  {
+    for (int k = threadIdx.x; k <= j; k += gm_threads)
+      lc[k] = h[k];
+    for (int k = threadIdx.x; k < j; k += gm_threads)
+    {
+      lcs[k] = cs[k];
+      lsn[k] = sn[k];
+    }
+  }
+  __syncthreads();
+  const P nrm = s_nrm;
+  const P inv = (nrm > P{0}) ? P{1} / nrm : P{0};
+  if (blockIdx.x == 0)
+  {
+    if (threadIdx.x == 0)
+    {
+      st->nrm = nrm;
+      st->inv_nrm = inv;
+      P x = lc[0];
+      for (int k = 0; k < j; k++)
+      {
+        P y = lc[k + 1];
+        dev_rot(x, y, lcs[k], lsn[k]);
+        lc[k] = x;
+        x = y;
+      }
+      P c, sv;
+      dev_rotg(x, nrm, c, sv);
+      lc[j] = x;
+      cs[j] = c;
+      sn[j] = sv;
+      const P res = fabs(sv * g[j]);
+      st->inner_res = res;
+      if (rotate_always != 0 || res > tol)
+      {
+        g[j + 1] = 0;
+        dev_rot(g[j], g[j + 1], c, sv);
+      }
+    }
+    __syncthreads();
+    P *col = H + (j * (j + 1)) / 2; // packed upper triangle, column j
+    for (int k = threadIdx.x; k <= j; k += gm_threads)
+      col[k] = lc[k];
+  }
+  if (inv == P{0})
+    return;
+  for (int64_t i = blockIdx.x * int64_t(gm_threads) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * gm_threads)
+    w[i] *= inv;
 }
 
 template<typename P>
@@ -511,11 +602,10 @@ struct gm_cycle
     const int ncols = j + 1;
     dim3 grid(nblk, (ncols + gm_cols - 1) / gm_cols);
     gm_dots<P><<<grid, gm_threads, 0, s>>>(n, rpb, V, ldv, w, ncols, dotp);
-    gm_reduce<P><<<ncols, gm_threads, 0, s>>>(nblk, dotp, h);
-    gm_update<P><<<nblk, gm_threads, sizeof(P) * ncols, s>>>(n, rpb, V, ldv, w, ncols, h, sqp);
-    gm_givens<P><<<1, 64, sizeof(P) * (3 * j + 1), s>>>(nblk, sqp, h, j, H, cs, sn, g, tol,
-                                                         rotate_always, st);
-    gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, w, st);
+    gm_reduce_update<P><<<nblk, gm_threads, sizeof(P) * ncols, s>>>(n, rpb, V, ldv, w, ncols,
+                                                                      dotp, h, sqp);
+    gm_givens_scale<P><<<sblocks, gm_threads, sizeof(P) * (3 * j + 1), s>>>(
+        n, w, nblk, sqp, h, j, H, cs, sn, g, tol, rotate_always, st);
     AKM_TRY(check_launch("gmres inner iteration"));
     AKM_HIP(hipMemcpyAsync(hst + (j & 1), st, sizeof(gm_state<P>), hipMemcpyDeviceToHost, s));
     AKM_HIP(hipEventRecord(ready[j & 1], s));
