@@ -41,6 +41,7 @@ namespace akm
 {
 constexpr int gk_max_dims = 6;
 constexpr int gk_threads = 256;
+constexpr int gk_group_max = 4; // chains sharing one first-stage input read
 
 enum gk_fill : int
 {
@@ -310,6 +311,135 @@ __global__ void __launch_bounds__(gk_threads)
   }
 }
 
+//! gk_cell_part for a group of chains whose stage reads the same input
+//! along the same dimension with the same fill (the first stage of chains of
+//! different terms): the input values are loaded once per connection and
+//! each chain's factor block is applied to them.  Every chain's sum is
+//! formed in the same order as gk_cell_part forms it.
+template<typename P, int NT, int GM>
+__device__ __forceinline__ void gk_cell_part_group(gk_args const &a, const P *__restrict__ vA,
+                                                   const int (&tg)[GM], int gm, int k, int fill,
+                                                   const P *__restrict__ in, int64_t cell,
+                                                   int lane, P (&acc)[GM])
+{
+  constexpr int U = 2;
+  const int N = a.N, d = a.d;
+  const int G = wave_size / N;
+  const int g = lane / N;
+  const int e = lane - g * N;
+  int stride_k = 1;
+  for (int j = k + 1; j < d; j++)
+    stride_k *= NT;
+  const int s = (e / stride_k) % NT;
+  const int e0 = e - s * stride_k;
+  const int dg = a.nb_diag[k][cell];
+  const int lo = (fill == gk_upper) ? dg : a.nb_ptr[k][cell];
+  const int hi = (fill == gk_lower) ? dg + 1 : a.nb_ptr[k][cell + 1];
+  const int64_t nn = int64_t(NT) * NT;
+  const int64_t tstride = int64_t(d) * nn;
+  const int64_t cstride = tstride * a.T;
+  const P *Ab[GM];
+#pragma unroll
+  for (int m = 0; m < GM; m++)
+  {
+    acc[m] = 0;
+    // members past gm repeat the first chain's (valid) blocks; their sums are dropped
+    Ab[m] = vA + int64_t(tg[m < gm ? m : 0]) * tstride + k * nn + s;
+  }
+  for (int base = lo; base < hi; base += wave_size)
+  {
+    const int pl = base + lane;
+    const int my_cj = (pl < hi) ? a.nb_cell[k][pl] : 0;
+    const int my_q = (pl < hi) ? a.nb_q[k][pl] : 0;
+    const int cnt = min(wave_size, hi - base);
+    const int iters = (cnt + G - 1) / G;
+    for (int it = 0; it < iters; it += U)
+    {
+      int cjv[U], qv[U], jlov[U], jhiv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+      {
+        const int r = (it + u) * G + g;
+        const int rr = min(r, wave_size - 1);
+        const int cj = __shfl(my_cj, rr, wave_size);
+        const int q = __shfl(my_q, rr, wave_size);
+        const bool ok = (it + u < iters) && g < G && r < cnt && cj < a.active_cells;
+        const bool diag = ok && (base + r == dg);
+        cjv[u] = ok ? cj : 0;
+        qv[u] = ok ? q : 0;
+        jlov[u] = (diag && fill == gk_upper) ? s : 0;
+        jhiv[u] = !ok ? 0 : ((diag && fill == gk_lower) ? s : NT);
+      }
+      P xv[U][NT];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+          xv[u][j] = in[int64_t(cjv[u]) * N + e0 + j * stride_k];
+#pragma unroll
+      for (int m = 0; m < GM; m++)
+      {
+        if (m >= gm) // the same on every lane
+          break;
+        P av[U][NT];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int j = 0; j < NT; j++)
+            av[u][j] = Ab[m][int64_t(qv[u]) * cstride + j * NT];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int j = 0; j < NT; j++)
+          {
+            const P v = fma(av[u][j], xv[u][j], acc[m]);
+            acc[m] = (j >= jlov[u] && j < jhiv[u]) ? v : acc[m];
+          }
+      }
+    }
+  }
+}
+
+//! The first stage in the cell layout with chains grouped by (dimension,
+//! fill): one wavefront per (group, cell) reads the neighbours' x values once
+//! for up to GM chains.  Group gi holds chains gchain[gptr[gi] .. gptr[gi+1]).
+template<typename P, int NT, int GM>
+__global__ void __launch_bounds__(gk_threads)
+    gk_cell_first(gk_args a, const P *__restrict__ vA, const int *__restrict__ gptr,
+                  const int *__restrict__ gchain, int ngroups, const P *__restrict__ x, P *work)
+{
+  __shared__ P red[gk_threads];
+  const int lane = threadIdx.x % wave_size;
+  P *wred = red + (threadIdx.x / wave_size) * wave_size;
+  const int64_t items = a.cells * ngroups;
+  for (int64_t w = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / wave_size; w < items;
+       w += int64_t(gk_threads / wave_size) * gridDim.x)
+  {
+    const int gi = static_cast<int>(w / a.cells);
+    const int g0 = gptr[gi], gm = gptr[gi + 1] - g0;
+    const int c0 = gchain[g0];
+    const int k = a.ch_dim[c0 * gk_max_dims];
+    const int fill = a.ch_fill[c0 * gk_max_dims];
+    const int64_t cell = a.order[k][w - int64_t(gi) * a.cells];
+    int tg[GM];
+#pragma unroll
+    for (int m = 0; m < GM; m++)
+      tg[m] = (m < gm) ? a.ch_term[gchain[g0 + m]] : 0;
+    P acc[GM];
+    gk_cell_part_group<P, NT, GM>(a, vA, tg, gm, k, fill, x, cell, lane, acc);
+#pragma unroll
+    for (int m = 0; m < GM; m++)
+    {
+      if (m < gm)
+      {
+        const P v = gk_cell_reduce(acc[m], wred, lane, a.N);
+        if (lane < a.N)
+          work[int64_t(a.ch_slot[gchain[g0 + m]]) * a.stride + cell * a.N + lane] = v;
+      }
+    }
+  }
+}
+
 //! y = (alpha * sum over chains of the chain's last stage (x for a term of
 //! identity factors), in chain order, + beta*y + gamma*xg) / (1 - jdt * pc)
 //! (no division when pc is null).
@@ -378,6 +508,11 @@ struct akm_global_s
   akm::dev_buffer<int> ch_term, ch_nst, ch_dim, ch_fill, ch_slot;
   std::vector<akm::dev_buffer<int>> stage_list; // per stage: the chains that have it
   std::vector<int> stage_count;
+  // first stage, cell layout: chains grouped by (dimension, fill), at most
+  // gk_group_max per group
+  akm::dev_buffer<int> first_ptr, first_chain;
+  int first_groups = 0, first_largest = 0;
+  bool first_grouped = false; // chosen by global_autotune
   int nchains = 0, nslots = 0, max_stages = 0;
   // lanes per DOF row: 1 (coalesced tensors), 8 (split lists), or 0: one
   // wavefront per cell (n^d <= 64)
@@ -448,7 +583,28 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
       continue;
     const int blocks = gk_blocks(lanes == 0 ? g.cells * nl * wave_size
                                             : g.cells * m.N * nl * lanes);
-    if (lanes == 0)
+    if (lanes == 0 && st == 0 && g.first_grouped && g.first_largest > 1 && m.n >= 2 &&
+        m.n <= 4)
+    {
+      // chains sharing their first-stage input read it once
+      const int fb = gk_blocks(g.cells * g.first_groups * wave_size);
+      const int *gp = g.first_ptr.data(), *gc = g.first_chain.data();
+      const int ng = g.first_groups;
+      const bool two = g.first_largest <= 2;
+      if (m.n == 2 && two)
+        gk_cell_first<P, 2, 2><<<fb, gk_threads, 0, m.stream>>>(a, vA, gp, gc, ng, x, work);
+      else if (m.n == 2)
+        gk_cell_first<P, 2, 4><<<fb, gk_threads, 0, m.stream>>>(a, vA, gp, gc, ng, x, work);
+      else if (m.n == 3 && two)
+        gk_cell_first<P, 3, 2><<<fb, gk_threads, 0, m.stream>>>(a, vA, gp, gc, ng, x, work);
+      else if (m.n == 3)
+        gk_cell_first<P, 3, 4><<<fb, gk_threads, 0, m.stream>>>(a, vA, gp, gc, ng, x, work);
+      else if (two)
+        gk_cell_first<P, 4, 2><<<fb, gk_threads, 0, m.stream>>>(a, vA, gp, gc, ng, x, work);
+      else
+        gk_cell_first<P, 4, 4><<<fb, gk_threads, 0, m.stream>>>(a, vA, gp, gc, ng, x, work);
+    }
+    else if (lanes == 0)
     {
       const int *list = g.stage_list[st].data();
       switch (m.n)
@@ -590,7 +746,14 @@ int global_device_jacobi(akm_matrix_s &m)
 
 int64_t global_flops(akm_matrix_s const &m) { return m.gk ? m.gk->flops : 0; }
 
-int global_lanes(akm_matrix_s const &m) { return m.gk ? m.gk->lanes : 0; }
+int global_layout(akm_matrix_s const &m)
+{
+  if (!m.gk)
+    return 0;
+  if (m.gk->lanes == 0)
+    return m.gk->first_grouped ? 3 : 2;
+  return m.gk->lanes;
+}
 
 int global_set_layout(akm_matrix_s &m, int layout)
 {
@@ -598,11 +761,12 @@ int global_set_layout(akm_matrix_s &m, int layout)
     return set_error(AKM_ERR_INVALID, "not a global kron matrix");
   if (layout == 0)
     return AKM_SUCCESS;
-  if (layout != 1 && layout != 2 && layout != 8)
-    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2 or 8", layout);
-  if (layout == 2 && m.N > wave_size)
+  if (layout != 1 && layout != 2 && layout != 3 && layout != 8)
+    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2, 3 or 8", layout);
+  if ((layout == 2 || layout == 3) && m.N > wave_size)
     return set_error(AKM_ERR_UNSUPPORTED, "global kron: the cell layout needs n^d <= 64");
-  m.gk->lanes = (layout == 2) ? 0 : layout;
+  m.gk->lanes = (layout == 2 || layout == 3) ? 0 : layout;
+  m.gk->first_grouped = (layout == 3);
   return AKM_SUCCESS;
 }
 
@@ -616,12 +780,21 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
   AKM_HIP(hipEventCreate(&e1));
   float best = -1;
   int best_lanes = 1;
+  bool best_grouped = false;
   int rc = AKM_SUCCESS;
-  for (int lanes : {1, 8, 0})
+  // layouts 1 and 8, then the cell layout with and without the grouped
+  // first stage (shared input reads against fewer, longer wavefronts)
+  const int shapes[4][2] = {{1, 0}, {8, 0}, {0, 0}, {0, 1}};
+  for (auto const &shape : shapes)
   {
+    const int lanes = shape[0];
+    const bool grouped = shape[1] != 0;
     if (lanes == 0 && m.N > wave_size)
       continue;
+    if (grouped && g.first_largest < 2)
+      continue;
     g.lanes = lanes;
+    g.first_grouped = grouped;
     rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
     if (rc != AKM_SUCCESS)
       break;
@@ -635,9 +808,11 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
     {
       best = ms;
       best_lanes = lanes;
+      best_grouped = grouped;
     }
   }
   g.lanes = best_lanes;
+  g.first_grouped = best_grouped;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
@@ -855,6 +1030,40 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
     g->stage_count[st] = static_cast<int>(list.size());
     if (!list.empty())
       AKM_TRY(g->stage_list[st].upload(list.data(), int64_t(list.size())));
+  }
+  if (g->max_stages > 0)
+  {
+    // first-stage groups: chains in order, grouped by (dimension, fill)
+    std::vector<std::pair<int, int>> keys;
+    std::vector<std::vector<int>> members;
+    for (int c = 0; c < g->nchains; c++)
+    {
+      if (ch_nst[c] < 1)
+        continue;
+      const std::pair<int, int> key(ch_dim[c * gk_max_dims], ch_fill[c * gk_max_dims]);
+      size_t i = 0;
+      while (i < keys.size() && keys[i] != key)
+        i++;
+      if (i == keys.size())
+      {
+        keys.push_back(key);
+        members.emplace_back();
+      }
+      members[i].push_back(c);
+    }
+    std::vector<int> gptr(1, 0), gchain;
+    for (auto const &mem : members)
+      for (size_t b = 0; b < mem.size(); b += gk_group_max)
+      {
+        for (size_t j = b; j < std::min(mem.size(), b + gk_group_max); j++)
+          gchain.push_back(mem[j]);
+        gptr.push_back(static_cast<int>(gchain.size()));
+      }
+    g->first_groups = static_cast<int>(gptr.size()) - 1;
+    for (int i = 0; i < g->first_groups; i++)
+      g->first_largest = std::max(g->first_largest, gptr[i + 1] - gptr[i]);
+    AKM_TRY(g->first_ptr.upload(gptr.data(), int64_t(gptr.size())));
+    AKM_TRY(g->first_chain.upload(gchain.data(), int64_t(gchain.size())));
   }
   if (nslots > 0)
     AKM_TRY(g->work.allocate(int64_t(2) * nslots * g->cells * m.N * int64_t(m.esize)));

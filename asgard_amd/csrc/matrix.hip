@@ -582,7 +582,7 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
     // row layout (reported in waves_per_row as akm_set_tuning takes it)
     AKM_TRY(global_autotune(*mat, x, y));
     if (waves_per_row)
-      *waves_per_row = (global_lanes(*mat) == 0) ? 2 : global_lanes(*mat);
+      *waves_per_row = global_layout(*mat);
     if (lds_tables)
       *lds_tables = 0;
     return AKM_SUCCESS;

@@ -579,7 +579,7 @@ def main():
     line["global_operator"] = {
         "us_per_apply": round(gmat.time_apply(1.0, xd, 0.0, gy, 50) * 1e3, 2),
         "pair_us_per_apply": round(ms_per_step * 1e3, 2),
-        "chains_stages": list(gmat.global_info()[1:]), "lanes_per_dof": glanes,
+        "chains_stages": list(gmat.global_info()[1:]), "layout": glanes,
         "max_rel_diff_vs_pair": float((gy - yd).abs().max().item() / yd.abs().max().item())}
     if solves:
         line["solve"] = {}

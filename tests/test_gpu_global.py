@@ -73,7 +73,7 @@ def _case_and_refs(spec):
     return _ORACLE[key]
 
 
-LAYOUTS = (1, 8, 2)  # lane per DOF, 8 lanes per DOF, wavefront per cell
+LAYOUTS = (1, 8, 2, 3)  # lane per DOF, 8 lanes per DOF, wavefront per cell (3: grouped first stage)
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
