@@ -34,6 +34,12 @@ constexpr int gm_threads = 256;
 constexpr int gm_cols = 8; // Krylov columns per dot-product block
 // the Givens kernel stages 3 * restart values in LDS (64 KB)
 constexpr int gm_max_restart = 2000;
+// host copies of the step state: Arnoldi steps in flight at once (the step
+// waited for plus the speculative ones) stay below this
+constexpr int gm_slots = 4;
+// speculative steps enqueued past the one waited for (two measured no faster
+// than one at d=4: the step is bound by the GPU's kernel chain, not the host)
+constexpr int gm_ahead = 1;
 
 template<typename P>
 struct gm_state
@@ -455,8 +461,8 @@ struct akm_gmres_workspace
   // flexible / mixed-precision solver only
   akm::dev_buffer<double> Z, xprev;
   akm::dev_buffer<float> v32, y32;
-  void *host_state = nullptr; // pinned, two slots
-  hipEvent_t state_ready[2] = {nullptr, nullptr};
+  void *host_state = nullptr; // pinned, gm_slots slots
+  hipEvent_t state_ready[akm::gm_slots] = {};
   ~akm_gmres_workspace()
   {
     if (host_state != nullptr)
@@ -607,30 +613,31 @@ struct gm_cycle
     gm_givens_scale<P><<<sblocks, gm_threads, sizeof(P) * (3 * j + 1), s>>>(
         n, w, nblk, sqp, h, j, H, cs, sn, g, tol, rotate_always, st);
     AKM_TRY(check_launch("gmres inner iteration"));
-    AKM_HIP(hipMemcpyAsync(hst + (j & 1), st, sizeof(gm_state<P>), hipMemcpyDeviceToHost, s));
-    AKM_HIP(hipEventRecord(ready[j & 1], s));
+    AKM_HIP(hipMemcpyAsync(hst + (j % gm_slots), st, sizeof(gm_state<P>),
+                           hipMemcpyDeviceToHost, s));
+    AKM_HIP(hipEventRecord(ready[j % gm_slots], s));
     return AKM_SUCCESS;
   }
 
   //! Arnoldi steps 0, 1, ... until the estimate reaches `stop` or `limit`
-  //! steps ran; count = the steps taken.  With `speculate`, step j + 1 is
-  //! enqueued before step j's estimate is read, which hides the host round
-  //! trip behind the GPU work.  When step j ends the cycle the speculative
-  //! step's column, Hessenberg column and rotation are never used (the
-  //! update reads columns and g entries below count only).
+  //! steps ran; count = the steps taken.  With `speculate`, steps j + 1 ..
+  //! j + gm_ahead are enqueued before step j's estimate is read, which keeps
+  //! the GPU busy through the host's round trips.  When step j ends the
+  //! cycle the speculative steps' columns, Hessenberg columns, rotations and
+  //! g entries are never used (the update reads columns and g entries below
+  //! count only), and the stream runs them before any later work.
   int run_steps(int limit, std::function<int(const P *, P *)> const &op,
                 std::function<int(const P *, P *)> const *pre, P tol, P stop, P &est,
                 int &count, bool speculate)
   {
+    static_assert(gm_ahead + 1 <= gm_slots, "steps in flight exceed the state slots");
     count = 0;
-    bool ahead = false;
+    int launched = 0;
+    const int depth = speculate ? gm_ahead + 1 : 1;
     while (est > stop && count < limit)
     {
-      if (!ahead)
-        AKM_TRY(launch_step(count, op, pre, tol));
-      ahead = speculate && count + 1 < limit;
-      if (ahead)
-        AKM_TRY(launch_step(count + 1, op, pre, tol));
+      while (launched < count + depth && launched < limit)
+        AKM_TRY(launch_step(launched++, op, pre, tol));
       AKM_TRY(wait_step(count, est));
       ++count;
     }
@@ -640,8 +647,8 @@ struct gm_cycle
   //! Waits for step j's state; est = its residual estimate.
   int wait_step(int j, P &est)
   {
-    AKM_HIP(hipEventSynchronize(ready[j & 1]));
-    est = hst[j & 1].inner_res;
+    AKM_HIP(hipEventSynchronize(ready[j % gm_slots]));
+    est = hst[j % gm_slots].inner_res;
     return AKM_SUCCESS;
   }
 
@@ -857,7 +864,8 @@ static int ensure_workspace(akm_matrix_s &m, int restart)
   AKM_TRY(ws.state.allocate(sizeof(gm_state<double>)));
   AKM_TRY(ws.xb.allocate(2 * n * int64_t(m.esize)));
   if (ws.host_state == nullptr)
-    AKM_HIP(hipHostMalloc(&ws.host_state, 2 * sizeof(gm_state<double>), hipHostMallocDefault));
+    AKM_HIP(hipHostMalloc(&ws.host_state, gm_slots * sizeof(gm_state<double>),
+                          hipHostMallocDefault));
   for (auto &e : ws.state_ready)
     if (e == nullptr)
       AKM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));

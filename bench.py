@@ -166,7 +166,7 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
         x = b.clone()
         info, t = timed(lambda: asgard_amd.simple_gmres_euler(dt, m64, x, b, restart, -1,
                                                               1e-10 * bnorm))
-    plain.update(seconds=round(t, 4), applies_f64=info.iterations, rel_residual=true_rel(x))
+    plain.update(seconds=round(t, 5), applies_f64=info.iterations, rel_residual=true_rel(x))
     x_plain = x
     out["plain_fp64"] = plain
 
@@ -177,7 +177,7 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
                                                                precond=pre, restart=restart,
                                                                tol=1e-10))
         rec = st.as_dict()
-        rec.update(seconds=round(t, 4), speedup_vs_plain=round(plain["seconds"] / t, 3),
+        rec.update(seconds=round(t, 5), speedup_vs_plain=round(plain["seconds"] / t, 3),
                    rel_residual_check=true_rel(xm),
                    rel_diff_vs_plain=float(torch.linalg.norm(xm - x_plain).item() /
                                            torch.linalg.norm(x_plain).item()))
@@ -208,14 +208,14 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
         ginfo, tg = timed(lambda: asgard_amd.simple_gmres_euler(dt, g64, xg, b, restart, -1,
                                                                 0.5e-10 * bnorm))
     out["global_plain_fp64_jacobi"] = dict(
-        seconds=round(tg, 4), iterations=ginfo.iterations, rel_residual=true_rel(xg),
+        seconds=round(tg, 5), iterations=ginfo.iterations, rel_residual=true_rel(xg),
         speedup_vs_plain=round(plain["seconds"] / tg, 3))
     for rep in range(2):
         xg = b.clone()
         gst, tg = timed(lambda: asgard_amd.gmres_euler_mixed(dt, g64, xg, b, mat32=g32,
                                                              restart=restart, tol=1e-10))
     rec = gst.as_dict()
-    rec.update(seconds=round(tg, 4), speedup_vs_plain=round(plain["seconds"] / tg, 3),
+    rec.update(seconds=round(tg, 5), speedup_vs_plain=round(plain["seconds"] / tg, 3),
                speedup_vs_global_plain=round(out["global_plain_fp64_jacobi"]["seconds"] / tg,
                                              3),
                rel_residual_check=true_rel(xg))
@@ -234,7 +234,7 @@ def solve_comparison(key, samples=200_000, epochs=8, with_cpu=True):
                                                              precond=pre, restart=restart,
                                                              tol=1e-10))
     rec = gst.as_dict()
-    rec.update(seconds=round(tg, 4), speedup_vs_plain=round(plain["seconds"] / tg, 3),
+    rec.update(seconds=round(tg, 5), speedup_vs_plain=round(plain["seconds"] / tg, 3),
                speedup_vs_global_plain=round(out["global_plain_fp64_jacobi"]["seconds"] / tg,
                                              3),
                rel_residual_check=true_rel(xg))
