@@ -60,10 +60,12 @@ GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
 class DgPartial(ctypes.Structure):
     """akm_dg_partial: one partial term of a 1D DG operator."""
     _fields_ = [("kind", ctypes.c_int), ("flux_scale", ctypes.c_double),
-                ("gq", ctypes.c_void_p), ("ge", ctypes.c_void_p)]
+                ("gq", ctypes.c_void_p), ("ge", ctypes.c_void_p),
+                ("left", ctypes.c_int), ("right", ctypes.c_int)]
 
 
-DG_MASS, DG_DIV, DG_GRAD = 0, 1, 2
+DG_MASS, DG_DIV, DG_GRAD, DG_PENALTY = 0, 1, 2, 3
+DG_PERIODIC, DG_DIRICHLET, DG_NEUMANN = 0, 1, 2
 
 #: akm_imex_hook and the events it runs at
 IMEX_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
@@ -517,9 +519,11 @@ def dg_quadrature(n, level, domain=(-1.0, 1.0), nq=None):
 def dg_term(n, level, partials, F, domain=(-1.0, 1.0), nq=None, stream=None):
     """generate_coefficients on the GPU for one 1D term (periodic domain).
 
-    partials: list of (kind, flux_scale, gq, ge) with kind DG_MASS / DG_DIV /
-    DG_GRAD and gq, ge FP64 CUDA tensors of the coefficient at the points of
-    :func:`dg_quadrature` (ge may be None for mass terms).  F: the multiwavelet
+    partials: list of (kind, flux_scale, gq, ge[, left, right]) with kind
+    DG_MASS / DG_DIV / DG_GRAD / DG_PENALTY, gq and ge FP64 CUDA tensors of the
+    coefficient at the points of :func:`dg_quadrature` (ge may be None for mass
+    terms, gq for penalty terms) and the boundary conditions DG_PERIODIC
+    (default) / DG_DIRICHLET / DG_NEUMANN at the two ends.  F: the multiwavelet
     transform as a CUDA tensor holding it column-major (``torch.as_tensor(F.T)``).
     Returns the operator as an N x N CUDA tensor holding it column-major, the
     form :meth:`KronmultMatrix.update_operators` takes."""
@@ -528,15 +532,19 @@ def dg_term(n, level, partials, F, domain=(-1.0, 1.0), nq=None, stream=None):
     nq = nq or max(10, n + 1)
     parts = (DgPartial * len(partials))()
     keep = []
-    for i, (kind, flux, gq, ge) in enumerate(partials):
-        gq = gq.contiguous()
-        keep.append(gq)
-        ptr_e = None
+    for i, part in enumerate(partials):
+        kind, flux, gq, ge = part[:4]
+        left, right = (part[4], part[5]) if len(part) > 4 else (DG_PERIODIC, DG_PERIODIC)
+        ptr_q = ptr_e = None
+        if gq is not None:
+            gq = gq.contiguous()
+            keep.append(gq)
+            ptr_q = gq.data_ptr()
         if ge is not None:
             ge = ge.contiguous()
             keep.append(ge)
             ptr_e = ge.data_ptr()
-        parts[i] = DgPartial(int(kind), float(flux), gq.data_ptr(), ptr_e)
+        parts[i] = DgPartial(int(kind), float(flux), ptr_q, ptr_e, int(left), int(right))
     Fc = F.contiguous()
     if Fc.shape != (N, N) or Fc.dtype != torch.float64 or not Fc.is_cuda:
         raise AkmInvalid(AKM_ERR_INVALID, "dg_term needs an N x N FP64 CUDA transform")

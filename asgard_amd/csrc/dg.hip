@@ -5,11 +5,13 @@
 //
 // A 1D term is the product of its partial terms (rechain_dimension), each a
 // mass, div or grad operator with its coefficient g given at the
-// Gauss-Legendre points of every cell and at the cell edges, on a periodic
-// domain [a, b] of 2^level cells:
+// Gauss-Legendre points of every cell and at the cell edges, on a domain
+// [a, b] of 2^level cells with periodic, Dirichlet or Neumann boundaries:
 //   mass: int g phi_j psi_i
 //   div: -int g phi_j psi_i' + numerical flux {{g u}} + s |g|/2 [[u]] on every edge
 //   grad: the negative transpose of a div with the flux direction reversed
+//         and the Dirichlet and Neumann roles swapped
+//   penalty: the jump part of the div flux only
 // in the cell basis of orthonormal Legendre polynomials (rows = test
 // functions), each multiplied by the inverse of the unit mass matrix, and the
 // product rotated to the multiwavelet basis: out = F (R_1 ... R_p) F^T with F
@@ -135,10 +137,12 @@ __global__ void __launch_bounds__(dg_threads) dg_fill_zero(int64_t count, double
 
 //! One partial term in the cell basis, column-major N x N (C zeroed): thread
 //! (cell i, row p) writes row n*i + p of the blocks (i, i-1), (i, i), (i, i+1)
-//! of the periodic domain.  kind 0 mass, 1 div.
+//! (neighbours wrap around on a periodic domain).  kind 0 mass, 1 div,
+//! 3 penalty (the jump part of the div flux only); lbc / rbc the boundary
+//! conditions at a and b (AKM_DG_PERIODIC, _DIRICHLET, _NEUMANN).
 __global__ void __launch_bounds__(dg_threads)
-    dg_assemble(const dg_basis *B, int n, int nq, int cells, int kind, double fscale,
-                const double *gq, const double *ge, double *C)
+    dg_assemble(const dg_basis *B, int n, int nq, int cells, int kind, double fscale, int lbc,
+                int rbc, const double *gq, const double *ge, double *C)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cells * n)
@@ -146,27 +150,54 @@ __global__ void __launch_bounds__(dg_threads)
   const int i = t / n, p = t - i * n;
   const int64_t N = int64_t(cells) * n;
   double left[dg_max_n], self[dg_max_n], right[dg_max_n];
+  const bool penalty = (kind == 3);
   for (int q = 0; q < n; q++)
   {
     double v = 0;
-    for (int k = 0; k < nq; k++)
-    {
-      const double c = gq[int64_t(i) * nq + k] * B->w[k];
-      v += (kind == 0 ? B->P[k * n + p] : -B->dP[k * n + p]) * c * B->P[k * n + q];
-    }
+    if (!penalty)
+      for (int k = 0; k < nq; k++)
+      {
+        const double c = gq[int64_t(i) * nq + k] * B->w[k];
+        v += (kind == 0 ? B->P[k * n + p] : -B->dP[k * n + p]) * c * B->P[k * n + q];
+      }
     self[q] = v;
     left[q] = right[q] = 0;
   }
-  if (kind == 1)
+  if (kind != 0)
   {
-    const double fl = 0.5 * ge[i], fr = 0.5 * ge[i + 1];
+    double fl = 0.5 * ge[i], fr = 0.5 * ge[i + 1];
     const double fla = fscale * fabs(fl), fra = fscale * fabs(fr);
+    if (penalty)
+      fl = fr = 0;
+    const bool first = (i == 0), last = (i == cells - 1);
     for (int q = 0; q < n; q++)
     {
-      left[q] += (-fl - fla) * B->PL[p] * B->PR[q];
-      self[q] += (-fl + fla) * B->PL[p] * B->PL[q];
-      self[q] += (fr + fra) * B->PR[p] * B->PR[q];
-      right[q] += (fr - fra) * B->PR[p] * B->PL[q];
+      // left edge: an interior or periodic edge couples to the left neighbour
+      if (!first || lbc == AKM_DG_PERIODIC)
+      {
+        left[q] += (-fl - fla) * B->PL[p] * B->PR[q];
+        self[q] += (-fl + fla) * B->PL[p] * B->PL[q];
+      }
+      else if (lbc == AKM_DG_DIRICHLET)
+      {
+        if (penalty)
+          self[q] += fla * B->PL[p] * B->PL[q];
+      }
+      else if (!penalty)
+        self[q] += -2.0 * fl * B->PL[p] * B->PL[q];
+      // right edge
+      if (!last || rbc == AKM_DG_PERIODIC)
+      {
+        self[q] += (fr + fra) * B->PR[p] * B->PR[q];
+        right[q] += (fr - fra) * B->PR[p] * B->PL[q];
+      }
+      else if (rbc == AKM_DG_DIRICHLET)
+      {
+        if (penalty)
+          self[q] += fra * B->PR[p] * B->PR[q];
+      }
+      else if (!penalty)
+        self[q] += 2.0 * fr * B->PR[p] * B->PR[q];
     }
   }
   const int il = (i + cells - 1) % cells, ir = (i + 1) % cells;
@@ -262,7 +293,10 @@ akm_status akm_dg_term(int n, int level, double a, double b, int nq,
   for (int i = 0; i < num_partials; i++)
   {
     const akm_dg_partial &pt = partials[i];
-    if (pt.kind < AKM_DG_MASS || pt.kind > AKM_DG_GRAD || pt.gq == nullptr ||
+    const bool bc_ok = pt.left >= AKM_DG_PERIODIC && pt.left <= AKM_DG_NEUMANN &&
+                       pt.right >= AKM_DG_PERIODIC && pt.right <= AKM_DG_NEUMANN;
+    if (pt.kind < AKM_DG_MASS || pt.kind > AKM_DG_PENALTY || !bc_ok ||
+        (pt.kind != AKM_DG_PENALTY && pt.gq == nullptr) ||
         (pt.kind != AKM_DG_MASS && pt.ge == nullptr))
       return set_error(AKM_ERR_INVALID, "dg term: partial term %d is malformed", i);
   }
@@ -284,10 +318,19 @@ akm_status akm_dg_term(int n, int level, double a, double b, int nq,
     const akm_dg_partial &pt = partials[i];
     dg_fill_zero<<<blocks, dg_threads, 0, s>>>(NN, C);
     AKM_TRY(check_launch("dg operator clear"));
-    // grad: a div with the flux direction reversed, transposed and negated
-    const double fs = (pt.kind == AKM_DG_GRAD) ? -pt.flux_scale : pt.flux_scale;
+    // grad: a div with the flux direction reversed and Dirichlet and Neumann
+    // swapped, transposed and negated
+    const bool grad = (pt.kind == AKM_DG_GRAD);
+    const double fs = grad ? -pt.flux_scale : pt.flux_scale;
+    auto swap_bc = [grad](int bc) {
+      if (!grad || bc == AKM_DG_PERIODIC)
+        return bc;
+      return bc == AKM_DG_DIRICHLET ? AKM_DG_NEUMANN : AKM_DG_DIRICHLET;
+    };
+    const int kind = (pt.kind == AKM_DG_MASS) ? 0 : (pt.kind == AKM_DG_PENALTY ? 3 : 1);
     dg_assemble<<<(cells * n + dg_threads - 1) / dg_threads, dg_threads, 0, s>>>(
-        basis.data(), n, nq, cells, pt.kind == AKM_DG_MASS ? 0 : 1, fs, pt.gq, pt.ge, C);
+        basis.data(), n, nq, cells, kind, fs, swap_bc(pt.left), swap_bc(pt.right), pt.gq, pt.ge,
+        C);
     AKM_TRY(check_launch("dg operator assembly"));
     // the first partial term goes straight to P, later ones are chained on
     double *dst = (i == 0) ? P : R;

@@ -236,31 +236,39 @@ akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, in
 
 /* --------------------------------------------------- DG operators (1D) -- */
 
-#define AKM_DG_MASS 0 /* int g phi_j psi_i */
-#define AKM_DG_DIV 1  /* -int g phi_j psi_i' + flux {{g u}} + s |g|/2 [[u]] */
-#define AKM_DG_GRAD 2 /* -(div with the flux direction reversed)^T */
+#define AKM_DG_MASS 0    /* int g phi_j psi_i */
+#define AKM_DG_DIV 1     /* -int g phi_j psi_i' + flux {{g u}} + s |g|/2 [[u]] */
+#define AKM_DG_GRAD 2    /* -(div, flux reversed, Dirichlet/Neumann swapped)^T */
+#define AKM_DG_PENALTY 3 /* the jump part s |g|/2 [[u]] of the div flux only */
+
+#define AKM_DG_PERIODIC 0
+#define AKM_DG_DIRICHLET 1 /* no boundary flux (penalty: the jump at the wall) */
+#define AKM_DG_NEUMANN 2   /* outflow flux at the wall */
 
 /* One partial term of a 1D DG operator.  gq: device array of the coefficient
  * g at the nq Gauss-Legendre points (ascending) of each of the 2^level cells,
- * cell-major; ge: device array of g at the 2^level + 1 cell edges a + e*h
- * (div and grad only).  flux_scale s: -1 downwind, 0 central, 1 upwind. */
+ * cell-major (not read for penalty terms); ge: device array of g at the
+ * 2^level + 1 cell edges a + e*h (all but mass terms).  flux_scale s: -1
+ * downwind, 0 central, 1 upwind.  left/right: the boundary conditions at a
+ * and b (partial_term::set_bilinear_boundary, src/pde/pde_base.hpp:240-270). */
 typedef struct
 {
   int kind;
   double flux_scale;
   const double *gq;
   const double *ge;
+  int left;
+  int right;
 } akm_dg_partial;
 
 /* generate_coefficients (src/coefficients.cpp:18-245) on the GPU for one
- * 1D term on a periodic domain [a, b] of 2^level cells with n basis
+ * 1D term on a domain [a, b] of 2^level cells with n basis
  * functions: the product of its partial terms (rechain_dimension), each
  * multiplied by the inverse unit mass matrix, rotated to the multiwavelet
  * basis, out = F (R_1 ... R_p) F^T.  F is the caller's N x N column-major
  * transform (operator_two_scale), out an N x N column-major device matrix,
  * N = n * 2^level <= 8192, ready for akm_update_operators.  Enqueued on
- * `stream` (hipStream_t, NULL = null stream) and synchronized.  No
- * counterpart for Dirichlet/Neumann boundaries or penalty terms. */
+ * `stream` (hipStream_t, NULL = null stream) and synchronized. */
 akm_status akm_dg_term(int n, int level, double a, double b, int nq,
                        const akm_dg_partial *partials, int num_partials, const double *F,
                        double *out, void *stream);

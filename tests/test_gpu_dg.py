@@ -3,8 +3,9 @@
 The host generator (asgard_amd/dg.py) is pinned against the reference's
 coefficient fixtures (tests/test_dg_operators.py).  Bar: the GPU operator in
 the multiwavelet basis agrees with dg.term_matrix to 1e-12 of max(1, |A|)
-(dense products in another order), for mass, div (every flux) and div-grad
-chains with constant and variable coefficients, n = 2, 3 and levels 0-5; a
+(dense products in another order), for mass, div (every flux), grad, penalty
+and div-grad chains with constant and variable coefficients, periodic,
+Dirichlet and Neumann boundaries, n = 2, 3 and levels 0-5; a
 sparse-grid handle updated from GPU-generated operators applies like one
 built from the host operators.
 """
@@ -20,7 +21,10 @@ from asgard_amd import dg  # noqa: E402
 from tests.kron_cases import sparse_grid_case  # noqa: E402
 
 FLUX = {"downwind": -1.0, "central": 0.0, "upwind": 1.0}
-KIND = {"mass": asgard_amd.DG_MASS, "div": asgard_amd.DG_DIV, "grad": asgard_amd.DG_GRAD}
+KIND = {"mass": asgard_amd.DG_MASS, "div": asgard_amd.DG_DIV, "grad": asgard_amd.DG_GRAD,
+        "penalty": asgard_amd.DG_PENALTY}
+BC = {"periodic": asgard_amd.DG_PERIODIC, "dirichlet": asgard_amd.DG_DIRICHLET,
+      "neumann": asgard_amd.DG_NEUMANN}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -41,7 +45,8 @@ def gpu_term(parts, n, level, domain=(-1.0, 1.0)):
         g = p["g"]
         gq = torch.from_numpy(np.vectorize(lambda x: g(x, 0.0))(pts).astype(np.float64)).cuda()
         ge = torch.from_numpy(np.array([g(x, 0.0) for x in edges], dtype=np.float64)).cuda()
-        args.append((KIND[p["kind"]], FLUX.get(p.get("flux", "central"), 0.0), gq, ge))
+        args.append((KIND[p["kind"]], FLUX.get(p.get("flux", "central"), 0.0), gq, ge,
+                     BC[p.get("left", "periodic")], BC[p.get("right", "periodic")]))
     F = cm(dg.wavelet_transform(n, level))
     out = asgard_amd.dg_term(n, level, args, F, domain)
     torch.cuda.synchronize()
@@ -57,6 +62,17 @@ CASES = {
                   dict(kind="grad", g=lambda x, t: 0.1, flux="central")],
     "field_mass": [dict(kind="mass", g=lambda x, t: 0.5 * np.cos(np.pi * x))],
     "grad_upwind": [dict(kind="grad", g=lambda x, t: 2.0 + np.cos(x), flux="upwind")],
+    "div_dirichlet": [dict(kind="div", g=lambda x, t: -1.0, flux="upwind", left="dirichlet",
+                           right="dirichlet")],
+    "div_neumann_mixed": [dict(kind="div", g=lambda x, t: 1.0 + 0.5 * x, flux="central",
+                               left="neumann", right="dirichlet")],
+    "diffusion_dirichlet": [dict(kind="div", g=lambda x, t: 1.0, flux="central",
+                                 left="dirichlet", right="dirichlet"),
+                            dict(kind="grad", g=lambda x, t: 0.1, flux="central",
+                                 left="dirichlet", right="dirichlet")],
+    "penalty_dirichlet": [dict(kind="penalty", g=lambda x, t: 1.0, flux="upwind",
+                               left="dirichlet", right="neumann")],
+    "penalty_periodic": [dict(kind="penalty", g=lambda x, t: 2.0, flux="upwind")],
 }
 
 
