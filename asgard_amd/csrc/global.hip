@@ -71,6 +71,7 @@ struct gk_args
   int nchains;
   int nslots;          // chains with at least one stage
   int64_t stride;      // work-buffer slot size (padded DOFs)
+  bool xcd_order;      // cell layouts: each XCD takes contiguous items
 };
 
 //! Lane g's share (connections g, g + LANES, ...) of one stage's product for
@@ -283,6 +284,20 @@ __device__ __forceinline__ P gk_cell_reduce(P v, P *red, int lane, int N)
   return sum;
 }
 
+//! Workgroup index with the workgroups of one XCD made consecutive: the
+//! hardware deals workgroups round-robin over the 8 XCDs, so workgroups
+//! b, b + 8, b + 16, ... share an L2.  With the grid a multiple of 8 this is
+//! a permutation, and the waves of one XCD take contiguous items (cells of
+//! the same fibres), whose neighbours then hit that XCD's L2.
+__device__ __forceinline__ int64_t gk_xcd_block(bool on)
+{
+  constexpr int xcds = 8;
+  const int per = gridDim.x / xcds;
+  return (on && per * xcds == static_cast<int>(gridDim.x))
+             ? int64_t(blockIdx.x % xcds) * per + blockIdx.x / xcds
+             : int64_t(blockIdx.x);
+}
+
 //! Stage `st` in the cell layout.
 template<typename P, int NT>
 __global__ void __launch_bounds__(gk_threads)
@@ -293,7 +308,7 @@ __global__ void __launch_bounds__(gk_threads)
   const int lane = threadIdx.x % wave_size;
   P *wred = red + (threadIdx.x / wave_size) * wave_size;
   const int64_t items = a.cells * nlist;
-  for (int64_t w = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / wave_size; w < items;
+  for (int64_t w = (gk_xcd_block(a.xcd_order) * gk_threads + threadIdx.x) / wave_size; w < items;
        w += int64_t(gk_threads / wave_size) * gridDim.x)
   {
     const int li = static_cast<int>(w / a.cells);
@@ -412,7 +427,7 @@ __global__ void __launch_bounds__(gk_threads)
   const int lane = threadIdx.x % wave_size;
   P *wred = red + (threadIdx.x / wave_size) * wave_size;
   const int64_t items = a.cells * ngroups;
-  for (int64_t w = (blockIdx.x * int64_t(gk_threads) + threadIdx.x) / wave_size; w < items;
+  for (int64_t w = (gk_xcd_block(a.xcd_order) * gk_threads + threadIdx.x) / wave_size; w < items;
        w += int64_t(gk_threads / wave_size) * gridDim.x)
   {
     const int gi = static_cast<int>(w / a.cells);
@@ -493,6 +508,13 @@ static int gk_blocks(int64_t work)
                                                                      gk_threads,
                                                                  65536)));
 }
+
+//! gk_blocks rounded up to a multiple of the 8 XCDs (for gk_xcd_block).
+static int gk_blocks_xcd(int64_t work)
+{
+  const int b = gk_blocks(work);
+  return (b + 7) / 8 * 8;
+}
 } // namespace akm
 
 //! Host and device data of a global operator.
@@ -513,6 +535,7 @@ struct akm_global_s
   akm::dev_buffer<int> first_ptr, first_chain;
   int first_groups = 0, first_largest = 0;
   bool first_grouped = false; // chosen by global_autotune
+  bool xcd_order = false;     // chosen by global_autotune (cell layouts)
   int nchains = 0, nslots = 0, max_stages = 0;
   // lanes per DOF row: 1 (coalesced tensors), 8 (split lists), or 0: one
   // wavefront per cell (n^d <= 64)
@@ -555,6 +578,7 @@ static gk_args global_args(akm_matrix_s const &m)
   a.nchains = g.nchains;
   a.nslots = g.nslots;
   a.stride = g.cells * m.N;
+  a.xcd_order = g.xcd_order;
   return a;
 }
 
@@ -581,13 +605,13 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
     const int nl = g.stage_count[st];
     if (nl == 0)
       continue;
-    const int blocks = gk_blocks(lanes == 0 ? g.cells * nl * wave_size
-                                            : g.cells * m.N * nl * lanes);
+    const int blocks = (lanes == 0) ? gk_blocks_xcd(g.cells * nl * wave_size)
+                                    : gk_blocks(g.cells * m.N * nl * lanes);
     if (lanes == 0 && st == 0 && g.first_grouped && g.first_largest > 1 && m.n >= 2 &&
         m.n <= 4)
     {
       // chains sharing their first-stage input read it once
-      const int fb = gk_blocks(g.cells * g.first_groups * wave_size);
+      const int fb = gk_blocks_xcd(g.cells * g.first_groups * wave_size);
       const int *gp = g.first_ptr.data(), *gc = g.first_chain.data();
       const int ng = g.first_groups;
       const bool two = g.first_largest <= 2;
@@ -751,7 +775,7 @@ int global_layout(akm_matrix_s const &m)
   if (!m.gk)
     return 0;
   if (m.gk->lanes == 0)
-    return m.gk->first_grouped ? 3 : 2;
+    return (m.gk->first_grouped ? 3 : 2) + (m.gk->xcd_order ? 2 : 0);
   return m.gk->lanes;
 }
 
@@ -761,17 +785,19 @@ int global_set_layout(akm_matrix_s &m, int layout)
     return set_error(AKM_ERR_INVALID, "not a global kron matrix");
   if (layout == 0)
     return AKM_SUCCESS;
-  if (layout != 1 && layout != 2 && layout != 3 && layout != 8)
-    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2, 3 or 8", layout);
-  if ((layout == 2 || layout == 3) && m.N > wave_size)
+  if (layout != 1 && layout != 8 && (layout < 2 || layout > 5))
+    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2, 3, 4, 5 or 8", layout);
+  const bool cell = (layout >= 2 && layout <= 5);
+  if (cell && m.N > wave_size)
     return set_error(AKM_ERR_UNSUPPORTED, "global kron: the cell layout needs n^d <= 64");
-  m.gk->lanes = (layout == 2 || layout == 3) ? 0 : layout;
-  m.gk->first_grouped = (layout == 3);
+  m.gk->lanes = cell ? 0 : layout;
+  m.gk->first_grouped = (layout == 3 || layout == 5);
+  m.gk->xcd_order = (layout == 4 || layout == 5);
   return AKM_SUCCESS;
 }
 
-//! Times both lane layouts on the operator itself (y is overwritten) and
-//! keeps the faster.
+//! Times the row layouts on the operator itself (y is overwritten) and
+//! keeps the fastest.
 int global_autotune(akm_matrix_s &m, const void *x, void *y)
 {
   akm_global_s &g = *m.gk;
@@ -782,19 +808,23 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
   int best_lanes = 1;
   bool best_grouped = false;
   int rc = AKM_SUCCESS;
+  bool best_xcd = false;
   // layouts 1 and 8, then the cell layout with and without the grouped
-  // first stage (shared input reads against fewer, longer wavefronts)
-  const int shapes[4][2] = {{1, 0}, {8, 0}, {0, 0}, {0, 1}};
+  // first stage (shared input reads against fewer, longer wavefronts), each
+  // cell layout with and without the XCD-contiguous workgroup order
+  const int shapes[6][3] = {{1, 0, 0}, {8, 0, 0}, {0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}};
   for (auto const &shape : shapes)
   {
     const int lanes = shape[0];
     const bool grouped = shape[1] != 0;
+    const bool xcd = shape[2] != 0;
     if (lanes == 0 && m.N > wave_size)
       continue;
     if (grouped && g.first_largest < 2)
       continue;
     g.lanes = lanes;
     g.first_grouped = grouped;
+    g.xcd_order = xcd;
     rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
     if (rc != AKM_SUCCESS)
       break;
@@ -809,10 +839,12 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
       best = ms;
       best_lanes = lanes;
       best_grouped = grouped;
+      best_xcd = xcd;
     }
   }
   g.lanes = best_lanes;
   g.first_grouped = best_grouped;
+  g.xcd_order = best_xcd;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;

@@ -91,14 +91,15 @@ int global_update(akm_matrix_s &m, const void *vA);
 int global_device_jacobi(akm_matrix_s &m);
 //! Global operator: flops per apply as the reference counts them.
 int64_t global_flops(akm_matrix_s const &m);
-//! Global operator: row layout in use, as akm_set_tuning takes it (1, 8, 2
-//! or 3).
+//! Global operator: row layout in use, as akm_set_tuning takes it (1, 8, 2,
+//! 3, 4 or 5).
 int global_layout(akm_matrix_s const &m);
 //! Global operator: times the lane layouts on the operator, keeps the faster.
 int global_autotune(akm_matrix_s &m, const void *x, void *y);
 //! Global operator: row layout 1 (lane per DOF), 8 (8 lanes per DOF), 2
 //! (wavefront per cell), 3 (wavefront per cell, chains sharing their first
-//! stage's input read it once), 0 (unchanged).
+//! stage's input read it once), 4 / 5 (2 / 3 with each XCD taking contiguous
+//! cells), 0 (unchanged).
 int global_set_layout(akm_matrix_s &m, int layout);
 
 //! Kernel arguments of an FP64 matrix (vectors and scalars left empty).

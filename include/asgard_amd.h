@@ -190,8 +190,9 @@ akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_
  * layout: 1 = one lane per DOF, 8 = eight lanes per DOF, 2 = one wavefront
  * per cell (n^d <= 64), 3 = as 2 with the chains whose first stage reads
  * the same input along the same dimension grouped (one read for up to four
- * chains), 0 = unchanged; akm_autotune times them and reports its choice the
- * same way.  No counterpart in the reference (its launch
+ * chains), 4 / 5 = as 2 / 3 with the workgroups of each XCD taking
+ * contiguous cells (neighbours shared in one L2), 0 = unchanged;
+ * akm_autotune times them and reports its choice the same way.  No counterpart in the reference (its launch
  * shape is fixed at build time, CMakeLists.txt:231-232). */
 akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
                           int force_generic);

@@ -73,7 +73,9 @@ def _case_and_refs(spec):
     return _ORACLE[key]
 
 
-LAYOUTS = (1, 8, 2, 3)  # lane per DOF, 8 lanes per DOF, wavefront per cell (3: grouped first stage)
+# lane per DOF, 8 lanes per DOF, wavefront per cell (3: grouped first stage;
+# 4, 5: 2, 3 with XCD-contiguous workgroups)
+LAYOUTS = (1, 8, 2, 3, 4, 5)
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -180,7 +182,7 @@ def test_layout_choice():
     ref = gk.apply(3, 2, c["table"], 4, 2, c["vA"], 1.0, c["x"], 0.0, c["y"])
     assert rel(host(y), ref) < TOL64
     with pytest.raises(asgard_amd.AkmError):
-        m.set_tuning(waves_per_row=4)
+        m.set_tuning(waves_per_row=6)
     big = grid_case(2, 9, 2, 1, seed=15)  # n^d = 81: no wavefront-per-cell layout
     mb = global_matrix(big)
     with pytest.raises(asgard_amd.AkmError):
