@@ -80,6 +80,10 @@ def bruteforce(case, alpha, beta):
 
 
 def grid_case(d, n, L, T, seed=0, refine=0.0, coupling="full", identity=()):
+    return grid_case_ops(d, n, L, T, seed, refine, coupling, identity)[0]
+
+
+def grid_case_ops(d, n, L, T, seed=0, refine=0.0, coupling="full", identity=()):
     """A sparse-grid case for the global operator: the |l|_1 <= L grid, or
     (refine > 0) the |l|_1 <= L-2 grid refined twice by asgard_amd.grid.refine
     up to level L; U[-1, 1] 1D operators, kept only on the volume connections
@@ -120,4 +124,4 @@ def grid_case(d, n, L, T, seed=0, refine=0.0, coupling="full", identity=()):
     active = [[0 if (t, k) in identity else 1 for k in range(d)] for t in range(T)]
     return dict(d=d, n=n, T=T, L=L, table=table, pntr=pntr, indx=indx, iA=iA, vA=vA,
                 x=rng.uniform(-1, 1, cells * N), y=rng.uniform(-1, 1, cells * N),
-                rows=cells, cols=cells, active=active)
+                rows=cells, cols=cells, active=active), ops

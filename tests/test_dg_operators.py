@@ -85,3 +85,62 @@ def test_rotated_blocks_follow_connect_1d():
         for c in range(M):
             if c not in conn:
                 assert np.abs(R[n * r:n * r + n, n * c:n * c + n]).max() < 1e-12, (r, c)
+
+
+# ---- penalty: pinned by the jump identity, independent of the assembly ----
+
+def _edge_values(u, n, level, domain):
+    a, b = domain
+    h = (b - a) / (1 << level)
+    PL = dg.legendre_onb([-1.0], n)[0][0] / np.sqrt(h)
+    PR = dg.legendre_onb([1.0], n)[0][0] / np.sqrt(h)
+    cells = u.reshape(-1, n)
+    return cells @ PL, cells @ PR
+
+
+@pytest.mark.parametrize("n,level", [(1, 3), (2, 4), (3, 3), (4, 2)])
+@pytest.mark.parametrize("bc", ["periodic", "dirichlet", "neumann"])
+@pytest.mark.parametrize("flux", ["upwind", "central", "downwind"])
+def test_penalty_jump_identity(n, level, bc, flux):
+    """u^T P u = sum over edges of s |g_e| / 2 [[u]]_e^2: interior edges
+    (and the wrap edge when periodic) use the jump between the two cells'
+    traces, a Dirichlet wall the trace itself, a Neumann wall nothing."""
+    domain = (-1.0, 2.0)
+    g = lambda x, t: 1.5 + np.cos(2 * np.pi * (x + 1.0) / 3.0)  # periodic on the domain
+    s = dg.FLUX[flux]
+    P = dg.cell_operator("penalty", n, level, domain, g, flux=flux, left=bc, right=bc)
+    rng = np.random.default_rng(level * 10 + n)
+    a, b = domain
+    cells = 1 << level
+    h = (b - a) / cells
+    for _ in range(3):
+        u = rng.standard_normal(n * cells)
+        uL, uR = _edge_values(u, n, level, domain)
+        want = 0.0
+        for i in range(1, cells):
+            want += s * abs(g(a + i * h, 0.0)) / 2 * (uL[i] - uR[i - 1]) ** 2
+        if bc == "periodic":
+            want += s * abs(g(a, 0.0)) / 2 * (uL[0] - uR[-1]) ** 2
+        elif bc == "dirichlet":
+            want += s * abs(g(a, 0.0)) / 2 * uL[0] ** 2 + s * abs(g(b, 0.0)) / 2 * uR[-1] ** 2
+        got = u @ P @ u
+        assert abs(got - want) <= 1e-12 * max(1.0, abs(want))
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_penalty_annihilates_continuous_functions(n):
+    """A continuous piecewise-linear periodic function has no jumps, so the
+    penalty maps it to zero (every row is a trace times a jump)."""
+    level, domain = 4, (0.0, 1.0)
+    cells = 1 << level
+    h = 1.0 / cells
+    nodes = np.random.default_rng(3).standard_normal(cells)  # values at x_i, periodic
+    # cell i: linear from nodes[i] to nodes[i+1]; projected on the Legendre basis
+    xq, wq = dg.gauss_legendre(8)
+    Pq = dg.legendre_onb(xq, n)[0] / np.sqrt(h)
+    u = np.zeros(n * cells)
+    for i in range(cells):
+        vals = nodes[i] + (nodes[(i + 1) % cells] - nodes[i]) * (xq + 1) / 2
+        u[i * n:(i + 1) * n] = Pq.T @ (vals * wq * h / 2)
+    P = dg.cell_operator("penalty", n, level, domain, lambda x, t: 2.0, flux="upwind")
+    assert np.abs(P @ u).max() <= 1e-12

@@ -122,28 +122,20 @@ def test_identity_factors_and_flux_direction():
     assert rel(host(y), ref) < TOL64
     _, chains, stages = m.global_info()
     assert chains == 2 + 1 + 1 and stages == 2  # term 2 is the identity
-    # flux direction: the reference's swap of active positions, applied in the oracle
-    c2 = grid_case(3, 2, 4, 1, seed=8, refine=0.3)
-    flux = [2]
-    act = [0, 1, 2]
-    act[0], act[2] = act[2], act[0]
-    m2 = global_matrix(c2, flux=flux)
+    # flux direction: the reference swaps it to the front of the active list;
+    # the expectation is the operator's literal definition (tests/global_dense.py)
+    from tests import global_dense
+    from tests.kron_cases import grid_case_ops
+
+    c2, ops2 = grid_case_ops(3, 2, 4, 1, seed=8, refine=0.3)
+    m2 = global_matrix(c2, flux=[2])
     y2 = dev(np.zeros_like(c2["x"]))
     m2.apply(1.0, dev(c2["x"]), 0.0, y2)
-    ops = gk.dof_operators(3, 2, 1, 4, c2["vA"])
-    elem = oracle.cell_1d_index(c2["table"])
-    pad = gk.padding(elem)
-    cells = np.concatenate([elem, np.asarray(pad, dtype=np.int64).reshape(-1, 3)])
-    dof = gk.dof_indices(cells, 2)
-    xp = np.zeros(dof.shape[0])
-    xp[:c2["x"].size] = c2["x"]
-    acc = np.zeros_like(xp)
-    for chain in gk.permutations(3):
-        w = xp
-        for pos, fill in chain:
-            w = gk._sweep(dof, act[pos], ops[0, act[pos]], fill, w)
-        acc += w
-    assert rel(host(y2), acc[:c2["x"].size]) < TOL64
+    want = global_dense.apply(3, 2, 4, 1, ops2, c2["table"], c2["x"], flux_dim=[2])
+    assert rel(host(y2), want) < TOL64
+    # the swap matters: without it the chains differ on this refined grid
+    plain = global_dense.apply(3, 2, 4, 1, ops2, c2["table"], c2["x"])
+    assert rel(plain, want) > 1e-6
 
 
 def test_no_terms_and_alpha_zero():

@@ -135,3 +135,34 @@ def test_jacobi_is_operator_diagonal():
     pc = gk.jacobi(d, n, c["table"], L, T, c["vA"])
     assert rel(pc, diag) < 1e-14
     assert np.allclose(gk.jacobi_scale(pc, 0.1, np.ones(size)), 1.0 / (1.0 - 0.1 * pc))
+
+
+# ---- refined grids with edge couplings: the literal definition ------------
+
+@pytest.mark.parametrize("d,n,L,T,frac,seed", [(2, 2, 5, 2, 0.3, 1), (3, 2, 4, 2, 0.3, 2),
+                                               (2, 3, 4, 1, 0.4, 3), (3, 2, 4, 1, 0.2, 4)])
+def test_refined_grid_edge_coupling_matches_definition(d, n, L, T, frac, seed):
+    """On refined grids with full (edge) coupling the global operator is not
+    the pair operator; it is fixed by its definition: per chain, the masked
+    1D operators' upper / full / lower parts applied along their dimensions
+    on the ancestry-closed cell set (tests/global_dense.py)."""
+    from tests import global_dense
+    from tests.kron_cases import grid_case_ops
+
+    c, ops = grid_case_ops(d, n, L, T, seed=seed, refine=frac)
+    got = gk.apply(d, n, c["table"], L, T, c["vA"], 1.0, c["x"], 0.0, c["y"])
+    want = global_dense.apply(d, n, L, T, ops, c["table"], c["x"])
+    assert rel(got, want) < 1e-13
+    # and it differs from the pair operator there (edge couplings across levels)
+    assert rel(got, pairwise(c, 1.0, 0.0)) > 1e-6
+
+
+def test_identity_and_flux_match_definition():
+    from tests import global_dense
+    from tests.kron_cases import grid_case_ops
+
+    ident = ((0, 1), (1, 0), (1, 2), (2, 0), (2, 1), (2, 2))
+    c, ops = grid_case_ops(3, 2, 4, 3, seed=4, refine=0.3, identity=ident)
+    got = gk.apply(3, 2, c["table"], 4, 3, c["vA"], 1.0, c["x"], 0.0, c["y"], active=c["active"])
+    want = global_dense.apply(3, 2, 4, 3, ops, c["table"], c["x"], active=c["active"])
+    assert rel(got, want) < 1e-13
