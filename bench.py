@@ -1,15 +1,19 @@
 """Benchmark: FP64 sparse-grid kronmult apply on one MI355X.
 
-Workload (one step = one apply y = A x over the whole grid, inputs resident
-in HBM): d = 3, n = 3 (degree 2), |l|_1 <= 6 sparse grid (688 cells, 18,576
-unknowns), 352,768 connected cell pairs, 3 terms with U[-1, 1] 1D operators of
-size 192 x 192 per dimension, x ~ N(0, 1), seed 0.
+Headline workload (one step = one apply y = A x over the whole grid, inputs
+resident in HBM): d = 6, n = 2 (degree 1), |l|_1 <= 7 sparse grid (16,172
+cells, 1,035,008 unknowns), 226,043,944 connected cell pairs, the 6 terms of
+a 3x+3v Vlasov advection operator (DG div/mass 1D operators, 256 x 256 per
+dimension), x ~ U[-1, 1], seed 0: 2,083 GFLOP per apply.
 
 Reported: GFLOP/s in the x2 convention of kronmult_matrix::compute_flops
 (2*d*n^(d+1)*T per connected pair and term), the roofline of the kronmult
-kernel (FP64 peak 78.6 TFLOP/s), the FP64 parity of the GPU result against
-the CPU oracle, and the oracle's OpenMP kronmult (a restatement of the
-reference's cpu_sparse) timed on this host's cores on the same apply.
+kernel (FP64 vector peak 78.6 TFLOP/s; the kernel issues no matrix-core
+instruction), the FP64 parity of the GPU result against the CPU oracle on the
+sampled rows, and the oracle's OpenMP kronmult (a restatement of the
+reference's cpu_sparse) timed on this host's cores over a bounded row sample
+of the same apply.  The d = 3, n = 3 |l|_1 <= 6 apply, FP32 applies, the global
+operator and the implicit-step solves are reported as extras.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -18,6 +22,11 @@ import json
 import os
 import sys
 import time
+
+# thread placement of the CPU baseline's OpenMP runtime; set before any
+# OpenMP library loads
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import numpy as np
 
@@ -47,44 +56,148 @@ def algorithmic_bytes(d, n, T, cells, nnz, nconn):
     return 8 * dof + 8 * dof + 4 * (cells + 1 + nnz) + 4 * cells * d + 8 * n * n * d * T * nconn
 
 
-def cpu_baseline(d, n, table, vA, x, T, L, ref_y_gpu):
+def cpu_threads():
+    """Threads of the CPU baseline: the OpenMP thread count the environment
+    grants this job (OMP_NUM_THREADS), else every core in the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_host_info(threads):
+    return {"cores": threads, "host_logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def cpu_baseline(d, n, table, vA, x, T, L, ref_y_gpu, reps=10):
+    """The oracle's OpenMP kronmult (restatement of the reference's
+    cpu_sparse) over the whole apply: 1 warm-up, median of `reps`."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     oracle.set_num_threads(threads)
     pntr, indx, iA = oracle.sparse_pattern(table, n=n, T=T, max_level=L)
     y = np.zeros_like(x)
     oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, y)  # warm-up
     times = []
-    for _ in range(20):
+    for _ in range(reps):
         t0 = time.perf_counter()
         oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, y)
         times.append(time.perf_counter() - t0)
     flops = 2 * d * n ** (d + 1) * T * int(pntr[-1])
     med = float(np.median(times))
     err = float(np.abs(ref_y_gpu - y).max() / np.abs(y).max())
-    return {"value": flops / med * 1e-9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": "full apply of the same workload, median of 20 after 1 warm-up "
-                      "(OpenMP restatement of the reference's cpu_sparse)"}, err
+    rec = {"value": round(flops / med * 1e-9, 2), "unit": "GFLOP/s", "kind": "port",
+           "sample": f"full apply of the same workload, median of {reps} after 1 warm-up "
+                     "(OpenMP restatement of the reference's cpu_sparse)"}
+    rec.update(cpu_host_info(threads))
+    return rec, err
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kron_f64_d3n3_L6_T3_hbm_traffic.txt")
+def cpu_baseline_rows(d, n, table, vA, x, T, L, y_gpu, stride=64, reps=10):
+    """The oracle's OpenMP kronmult over a bounded sample of the apply: every
+    `stride`-th row cell of the element table (rows of every length, in table
+    order), all of each row's connected columns and terms; 1 warm-up, median
+    of `reps`.  Also the FP64 parity of the GPU rows against it."""
+    import oracle
+
+    threads = cpu_threads()
+    oracle.set_num_threads(threads)
+    cells = table.shape[0]
+    N = n ** d
+    rows = np.arange(0, cells, stride, dtype=np.int32)
+    pntr, indx, iA = oracle.sparse_pattern_rows(table, rows, n=n, T=T, max_level=L)
+    y = np.zeros(len(rows) * N)
+    oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, y)  # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, y)
+        times.append(time.perf_counter() - t0)
+    pairs = int(pntr[-1])
+    flops = 2 * d * n ** (d + 1) * T * pairs
+    med = float(np.median(times))
+    gpu_rows = y_gpu.reshape(cells, N)[rows].reshape(-1)
+    err = float(np.abs(gpu_rows - y).max() / np.abs(y).max())
+    rec = {"value": round(flops / med * 1e-9, 2), "unit": "GFLOP/s", "kind": "port",
+           "sample": f"every {stride}th row cell of the element table: {len(rows)} of {cells} "
+                     f"rows, {pairs:,} of the apply's connected pairs, all {T} terms, "
+                     f"{flops / 1e9:.1f} GFLOP; 1 warm-up, median of {reps} applies "
+                     f"({med:.2f} s each) of the OpenMP restatement of the reference's "
+                     "cpu_sparse",
+           "seconds_per_sample_apply": round(med, 4)}
+    rec.update(cpu_host_info(threads))
+    return rec, err
 
 
-def profiled_traffic(waves, lds_tables):
-    """HBM bytes per apply of the chosen kernel shape, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (each counter
-    scaled by what it reads for a plain device copy of known size); None when
-    the shape was not profiled."""
+def profiled_traffic(profile, waves, lds_tables):
+    """HBM bytes per apply of the chosen kernel shape, from a committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (each counter scaled by what it
+    reads for a plain device copy of known size); None when the shape was not
+    profiled."""
     key = "traffic_bytes W=%d lds=%d:" % (waves, 1 if lds_tables else 0)
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(os.path.join(ROOT, "profiles", profile)) as f:
             for line in f:
                 if line.startswith(key):
                     return int(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+HEADLINE = dict(d=6, n=2, L=7, profile="kron_f64_d6n2_L7_T6_hbm_traffic.txt",
+                workload="kronmult apply, d=6 n=2 |l|_1<=7 sparse grid, 6 terms "
+                         "(3x+3v Vlasov advection operators)")
+SMALL = dict(d=3, n=3, L=6, T=3, profile="kron_f64_d3n3_L6_T3_hbm_traffic.txt",
+             workload="kronmult apply, d=3 n=3 |l|_1<=6 sparse grid, 3 terms, U[-1,1] "
+                      "factors")
+
+
+def headline_workload(seed=0):
+    """The d=6 n=2 |l|_1<=7 grid with the Vlasov operator's 6 terms and
+    x ~ U[-1, 1]."""
+    from asgard_amd import dg, grid
+
+    d, n, L = HEADLINE["d"], HEADLINE["n"], HEADLINE["L"]
+    table, _ = grid.element_table([L] * d)
+    ops = dg.operators_vlasov(n, L)
+    T = len(ops) // d
+    vA = grid.build_vA(d, n, T, L, ops)
+    x = np.random.default_rng(seed).uniform(-1.0, 1.0, table.shape[0] * n ** d)
+    return d, n, L, T, table, vA, x
+
+
+def time_steps(mat, xd, yd, steps, warmup, stream):
+    """W untimed then K timed applies between HIP events recorded on the
+    stream the kernel is launched on; returns (ms per apply, wall seconds)."""
+    import torch
+
+    for _ in range(warmup):
+        mat.apply(1.0, xd, 0.0, yd)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(steps):
+        mat.apply(1.0, xd, 0.0, yd)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(stop) / steps, wall
+
+
+def roofline(flops, ms, peak, profile, waves, lds, alg_bytes):
+    achieved = flops / (ms * 1e-3) * 1e-12
+    return {"bound": "valu_fp64" if peak == FP64_PEAK_TFLOPS else "valu_fp32",
+            "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": profiled_traffic(profile, waves, lds) if profile else None,
+            "algorithmic_bytes": int(alg_bytes)}
 
 
 SOLVE_WORKLOADS = {
@@ -345,11 +458,13 @@ def global_operator_applies(reps=20):
     return out
 
 
-def batch_systems(count=64, seed=0):
+def batch_systems(count=64, seed=0, inputs=None):
     """The batched-solve workload: `count` implicit steps, d ~ U{2..5}, n in
     {2, 3}, each on the |l|_1 <= 5 sparse grid with 10-30% of its leaf cells
     refined one level (seeded), d terms (advection in the first d/2
-    dimensions, diffusion nu = 0.1 in the others), dt = 1e-3, b ~ U[-1, 1]."""
+    dimensions, diffusion nu = 0.1 in the others), dt = 1e-3, b ~ U[-1, 1].
+    When `inputs` is a list, (d, n, table, max level, vA) of every system is
+    appended to it."""
     import asgard_amd
     from asgard_amd import dg, grid
 
@@ -365,6 +480,8 @@ def batch_systems(count=64, seed=0):
         ops = dg.operators_advection_diffusion(d, n, L, n_adv=max(1, d // 2), nu=0.1)
         vA = grid.build_vA(d, n, d, L, ops)
         mats.append(asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, d, vA))
+        if inputs is not None:
+            inputs.append((d, n, table, L, vA))
         dts.append(1e-3)
         bs.append(rng.uniform(-1.0, 1.0, table.shape[0] * n ** d))
     return mats, dts, bs
@@ -456,12 +573,104 @@ def calibrate_copy(mib=512, reps=5):
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}))
 
 
+def small_apply(stream, no_cpu, steps=100, warmup=10):
+    """The d=3 n=3 |l|_1<=6 apply with 3 random terms: its GFLOP/s, roofline
+    and parity against the oracle's full apply, and its DG / FP32 variants."""
+    import torch
+
+    import asgard_amd
+
+    d, n, L, T = SMALL["d"], SMALL["n"], SMALL["L"], SMALL["T"]
+    table, vA, x = make_workload(d, n, L, T)
+    mat = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
+    mat.set_stream(stream.cuda_stream)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    waves, lds = mat.autotune(xd, yd)
+    ms, _ = time_steps(mat, xd, yd, steps, warmup, stream)
+    nconn = len(asgard_amd.connect_1d(L)[1])
+    rec = {"workload": SMALL["workload"], "cells": int(table.shape[0]),
+           "connected_pairs": int(mat.num_pairs()), "flops_per_apply": int(mat.flops()),
+           "ms_per_apply": round(ms, 5), "gflops": round(mat.flops() / (ms * 1e-3) * 1e-9, 1),
+           "waves_per_row": waves, "lds_factor_tables": lds,
+           "roofline": roofline(mat.flops(), ms, FP64_PEAK_TFLOPS, SMALL["profile"], waves, lds,
+                                algorithmic_bytes(d, n, T, table.shape[0], mat.num_pairs(),
+                                                  nconn))}
+    if not no_cpu:
+        cpu, parity = cpu_baseline(d, n, table, vA, x, T, L, yd.cpu().numpy())
+        rec["cpu_baseline"] = cpu
+        rec["parity_max_rel_err_vs_oracle"] = parity
+    rec["variants"] = workload_variants(d, n, L, T, table, x, yd, stream, no_cpu)
+    gmat = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA)
+    gmat.set_stream(stream.cuda_stream)
+    glanes = gmat.autotune(xd, torch.zeros_like(xd))[0]
+    gy = torch.zeros_like(xd)
+    gmat.apply(1.0, xd, 0.0, gy)
+    torch.cuda.synchronize()
+    rec["global_operator"] = {
+        "us_per_apply": round(gmat.time_apply(1.0, xd, 0.0, gy, 50) * 1e3, 2),
+        "chains_stages": list(gmat.global_info()[1:]), "layout": glanes,
+        "max_rel_diff_vs_pair": float((gy - yd).abs().max().item() / yd.abs().max().item())}
+    return rec
+
+
+def headline_fp32(d, n, L, T, table, vA, xd, y64, stream):
+    """The FP32 kronmult of the headline workload: GFLOP/s, FP32 roofline and
+    its error against the FP64 GPU result."""
+    import torch
+
+    import asgard_amd
+
+    m32 = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA.astype(np.float32),
+                                                dtype=np.float32)
+    m32.set_stream(stream.cuda_stream)
+    x32 = xd.float()
+    y32 = torch.zeros_like(x32)
+    waves, lds = m32.autotune(x32, y32)
+    ms = m32.time_apply(1.0, x32, 0.0, y32, 5)
+    m32.apply(1.0, x32, 0.0, y32)
+    err = float(((y32.double() - y64).abs().max() / y64.abs().max()).item())
+    ach = m32.flops() / (ms * 1e-3) * 1e-12
+    return {"ms_per_apply": round(ms, 4), "gflops": round(ach * 1e3, 1),
+            "waves_per_row": waves, "lds_factor_tables": lds,
+            "roofline_frac_fp32": round(ach / FP32_PEAK_TFLOPS, 4),
+            "max_rel_err_vs_fp64": err}
+
+
+def solve_targets(solve):
+    """Time-to-1e-10 ratios of the implicit step, FP32 kronmult + learned
+    preconditioner against plain FP64 GMRES on the same operator (the pair
+    operator, and the global operator), per solve workload."""
+    out = {}
+    for k, rec in solve.items():
+        if "plain_fp64" not in rec:
+            continue
+        r = {}
+        if "fp32_refinement_learned" in rec:
+            r["pair_fp64_plain_s"] = rec["plain_fp64"]["seconds"]
+            r["pair_fp32_learned_s"] = rec["fp32_refinement_learned"]["seconds"]
+            r["pair_ratio"] = round(r["pair_fp64_plain_s"] / r["pair_fp32_learned_s"], 3)
+            r["pair_fp32_no_network_s"] = rec["fp32_refinement"]["seconds"]
+        if "global_fp32_refinement_learned" in rec:
+            r["global_fp64_plain_s"] = rec["global_plain_fp64_jacobi"]["seconds"]
+            r["global_fp32_learned_s"] = rec["global_fp32_refinement_learned"]["seconds"]
+            r["global_ratio"] = round(r["global_fp64_plain_s"] / r["global_fp32_learned_s"], 3)
+        r["target"] = 4.0
+        r["met"] = bool(max(r.get("pair_ratio", 0), r.get("global_ratio", 0)) >= 4.0)
+        out[k] = r
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline apply only (no d=3 apply, FP32, global operator)")
+    ap.add_argument("--small-only", action="store_true",
+                    help="print only the d=3 n=3 apply record (one JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 512 MiB device copy (HBM counter calibration)")
     ap.add_argument("--solve", default="d4n2,d6n2,batch64",
@@ -474,6 +683,11 @@ def main():
     ap.add_argument("--samples", type=int, default=200_000,
                     help="per-cell training samples of the learned preconditioner")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (the row-sharded multi-GPU design is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -481,6 +695,7 @@ def main():
         print(json.dumps({"global_operator_applies": global_operator_applies()}), flush=True)
         return
     solves = [] if args.solve == "none" else [k for k in args.solve.split(",") if k]
+
     def run_solve(k):
         if k == "batch64":
             return batch_comparison(64)
@@ -490,21 +705,19 @@ def main():
         for k in solves:
             print(json.dumps({"solve": k, **run_solve(k)}), flush=True)
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
-              "supported (the row-sharded multi-GPU design is described in DESIGN.md)",
-              file=sys.stderr)
-        sys.exit(2)
 
     import torch
 
     import asgard_amd
 
-    d, n, L, T = 3, 3, 6, 3
-    table, vA, x = make_workload(d, n, L, T)
+    stream = torch.cuda.current_stream()
+    if args.small_only:
+        print(json.dumps(small_apply(stream, args.no_cpu)), flush=True)
+        return
+
+    d, n, L, T, table, vA, x = headline_workload()
     cells = table.shape[0]
     mat = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
-    stream = torch.cuda.current_stream()
     mat.set_stream(stream.cuda_stream)
     xd = torch.from_numpy(x).cuda()
     yd = torch.zeros_like(xd)
@@ -513,32 +726,14 @@ def main():
     nconn = len(asgard_amd.connect_1d(L)[1])
     # setup, untimed: the autotuner picks the workgroup shape and LDS layout
     waves, lds_tables = mat.autotune(xd, yd)
-
-    for _ in range(args.warmup):
-        mat.apply(1.0, xd, 0.0, yd)
-    torch.cuda.synchronize()
-
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        mat.apply(1.0, xd, 0.0, yd)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = start.elapsed_time(stop)
-    ms_per_step = ev_ms / args.steps
+    ms_per_step, wall = time_steps(mat, xd, yd, args.steps, args.warmup, stream)
     gflops = flops / (ms_per_step * 1e-3) * 1e-9
 
-    y_gpu = yd.cpu().numpy()
     cpu = None
     parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(d, n, table, vA, x, T, L, y_gpu)
+        cpu, parity = cpu_baseline_rows(d, n, table, vA, x, T, L, yd.cpu().numpy())
 
-    achieved_tflops = gflops * 1e-3
     line = {
         "metric": "kronmult GFLOP/s (FP64 apply)",
         "value": round(gflops, 2),
@@ -553,41 +748,32 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "kronmult apply, d=3 n=3 |l|_1<=6 sparse grid, 3 terms",
-                   "cells": int(cells), "connected_pairs": int(nnz),
-                   "flops_per_apply": int(flops), "parallelism": "single GPU",
-                   "waves_per_row": waves, "lds_factor_tables": lds_tables},
-        "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
-                     "traffic": profiled_traffic(waves, lds_tables),
-                     "algorithmic_bytes": int(algorithmic_bytes(d, n, T, cells, nnz, nconn))},
+        "config": {"workload": HEADLINE["workload"], "d": d, "n": n, "max_level": L,
+                   "terms": T, "cells": int(cells), "unknowns": int(mat.output_size()),
+                   "connected_pairs": int(nnz), "flops_per_apply": int(flops),
+                   "parallelism": "single GPU", "waves_per_row": waves,
+                   "lds_factor_tables": lds_tables},
+        "roofline": roofline(flops, ms_per_step, FP64_PEAK_TFLOPS, HEADLINE["profile"], waves,
+                             lds_tables, algorithmic_bytes(d, n, T, cells, nnz, nconn)),
         "cpu_baseline": cpu,
         "parity_max_rel_err_vs_oracle": parity,
     }
-    # the workload's other variants: the DG operators (advection in one
-    # dimension per term, mass elsewhere) and the FP32 kronmult
-    line["variants"] = workload_variants(d, n, L, T, table, x, yd, stream, args.no_cpu)
-
-    # the same workload through the global operator (every factor active here)
-    gmat = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA)
-    gmat.set_stream(stream.cuda_stream)
-    glanes = gmat.autotune(xd, torch.zeros_like(xd))[0]
-    gy = torch.zeros_like(xd)
-    gmat.apply(1.0, xd, 0.0, gy)
-    torch.cuda.synchronize()
-    line["global_operator"] = {
-        "us_per_apply": round(gmat.time_apply(1.0, xd, 0.0, gy, 50) * 1e3, 2),
-        "pair_us_per_apply": round(ms_per_step * 1e3, 2),
-        "chains_stages": list(gmat.global_info()[1:]), "layout": glanes,
-        "max_rel_diff_vs_pair": float((gy - yd).abs().max().item() / yd.abs().max().item())}
+    print(json.dumps({"progress": "headline done", "value": line["value"]}), file=sys.stderr,
+          flush=True)
+    if not args.no_extras:
+        line["fp32_apply"] = headline_fp32(d, n, L, T, table, vA, xd, yd, stream)
+        line["variants"] = {"d3n3_L6_T3": small_apply(stream, args.no_cpu)}
+    del mat
+    torch.cuda.empty_cache()
     if solves:
         line["solve"] = {}
         for k in solves:
+            print(json.dumps({"progress": f"solve {k}"}), file=sys.stderr, flush=True)
             try:
                 line["solve"][k] = run_solve(k)
             except Exception as e:  # report, keep the kronmult line
                 line["solve"][k] = {"error": f"{type(e).__name__}: {e}"}
+        line["solve_target_4x"] = solve_targets(line["solve"])
     print(json.dumps(line))
 
 
