@@ -48,7 +48,7 @@ EXPORTS = (
     "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
     "akm_update_operators", "akm_imex_step", "akm_dg_term",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
-    "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
+    "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler", "akm_gmres_dense",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
     "akm_mlp_get_params", "akm_mlp_set_params", "akm_mlp_forward", "akm_mlp_gradients",
     "akm_mlp_adam_step", "akm_mlp_train", "akm_precond_apply", "akm_precond_destroy",
@@ -144,6 +144,8 @@ def load_library(path: str = LIB_PATH):
     L.akm_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]
     L.akm_destroy.argtypes = [_vp]
     L.akm_gmres_euler.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
+                                  ctypes.POINTER(_i)]
+    L.akm_gmres_dense.argtypes = [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
                                   ctypes.POINTER(_i)]
     L.akm_gmres_euler_mixed.argtypes = [_vp, _vp, _vp, _d, _vp, _vp, _i, _i, _d, _d,
                                         ctypes.POINTER(SolveStats)]
@@ -570,6 +572,37 @@ def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, t
     _check(load_library().akm_gmres_euler(mat._h, float(dt), px, pb, int(restart),
                                           int(max_iter), float(tol), dx, ctypes.byref(err),
                                           ctypes.byref(its)), allow_not_converged=True)
+    return GmresInfo(err.value, its.value)
+
+
+def simple_gmres(A, x, b, M=None, restart=-1, max_iter=-1, tol=-1.0):
+    """solver::simple_gmres on an explicit matrix (src/solver.cpp:54-74): solves
+    A x = b in place for host numpy arrays (float64 or float32; A and M n x n,
+    M an optional left preconditioner applied through its LU factors);
+    returns GmresInfo(error, iterations).  Defaults and the absolute tolerance
+    follow simple_gmres_euler; raises ValueError when restart exceeds n."""
+    x_arr = x
+    if not isinstance(x_arr, np.ndarray) or x_arr.dtype not in (np.float64, np.float32) \
+            or not x_arr.flags.c_contiguous:
+        raise AkmInvalid(AKM_ERR_INVALID, "x must be a contiguous float64/float32 numpy array")
+    dtype = x_arr.dtype.type
+    n = x_arr.size
+    Af = np.asfortranarray(A, dtype=dtype)
+    if Af.shape != (n, n):
+        raise AkmInvalid(AKM_ERR_INVALID, "A must be n x n")
+    bb = _np(b, dtype)
+    Mf = None
+    if M is not None:
+        Mf = np.asfortranarray(M, dtype=dtype)
+        if Mf.shape != (n, n):
+            raise AkmInvalid(AKM_ERR_INVALID, "M must be n x n")
+    err = _d()
+    its = _i()
+    _check(load_library().akm_gmres_dense(
+        int(n), Af.ctypes.data, x_arr.ctypes.data, bb.ctypes.data,
+        None if Mf is None else Mf.ctypes.data, int(restart), int(max_iter), float(tol),
+        AKM_F64 if dtype == np.float64 else AKM_F32, ctypes.byref(err), ctypes.byref(its)),
+        allow_not_converged=True)
     return GmresInfo(err.value, its.value)
 
 

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <functional>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 namespace akm
@@ -448,6 +449,67 @@ gm_axpy_f32(int64_t n, const double *v, double c, const float *y, double *w)
     w[i] = fma(c, static_cast<double>(y[i]), v[i]);
 }
 
+//! y = alpha*A*x + beta*y for a dense n x n column-major A, one thread per
+//! row (consecutive threads read consecutive entries of a column); beta == 0
+//! never reads y.
+template<typename P>
+__global__ void __launch_bounds__(gm_threads)
+gm_dense_gemv(int n, const P *A, const P *x, P alpha, P beta, P *y)
+{
+  const int i = blockIdx.x * gm_threads + threadIdx.x;
+  if (i >= n)
+    return;
+  P s = 0;
+  for (int j = 0; j < n; j++)
+    s = fma(A[int64_t(j) * n + i], x[j], s);
+  y[i] = (beta == P{0}) ? alpha * s : fma(alpha, s, beta * y[i]);
+}
+
+//! v = M^-1 v from the LU factors of M with partial pivoting (getrs after
+//! getrf): the row interchanges piv in order, then the unit lower and the
+//! upper triangular solves, column by column.  One workgroup, v in LDS.
+template<typename P>
+__global__ void __launch_bounds__(256) gm_lu_solve(int n, const P *LU, const int *piv, P *v)
+{
+  extern __shared__ unsigned char lu_raw[];
+  P *lv = reinterpret_cast<P *>(lu_raw);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += 256)
+    lv[i] = v[i];
+  __syncthreads();
+  if (tid == 0)
+    for (int k = 0; k < n; k++)
+    {
+      const int p = piv[k];
+      if (p != k)
+      {
+        const P t = lv[k];
+        lv[k] = lv[p];
+        lv[p] = t;
+      }
+    }
+  __syncthreads();
+  for (int k = 0; k < n; k++)
+  {
+    const P vk = lv[k];
+    for (int i = k + 1 + tid; i < n; i += 256)
+      lv[i] -= LU[int64_t(k) * n + i] * vk;
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; k--)
+  {
+    if (tid == 0)
+      lv[k] /= LU[int64_t(k) * n + k];
+    __syncthreads();
+    const P vk = lv[k];
+    for (int i = tid; i < k; i += 256)
+      lv[i] -= LU[int64_t(k) * n + i] * vk;
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += 256)
+    v[i] = lv[i];
+}
+
 } // namespace akm
 
 //! Device workspace of the solvers, kept with the matrix between solves.
@@ -512,6 +574,42 @@ static int timed_on_stream(hipStream_t s, F &&f, double &ms)
   return rc;
 }
 
+//! Memory of one solve: the Krylov basis V (n x (restart + 1)), the small
+//! work array (h, packed H, cs, sn, g, dot and norm partials:
+//! gm_work_values of them), the device step state, gm_slots host copies of
+//! it, and gm_slots events marking those copies (or null: the host then
+//! waits for the whole stream).  Every value is written by a kernel before
+//! any kernel or the host reads it, so none needs clearing between solves.
+struct gm_memory
+{
+  void *V = nullptr;
+  void *work = nullptr;
+  void *state = nullptr;
+  void *host_state = nullptr;
+  hipEvent_t *ready = nullptr;
+};
+
+//! Values in the small work array of a solve.
+static int64_t gm_work_values(int64_t n, int restart)
+{
+  const int64_t rpb = rows_per_block_for(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
+  return 4 * int64_t(restart + 2) + int64_t(restart) * (restart + 1) / 2 + 2 +
+         int64_t(restart + 1) * nblk + nblk + 16;
+}
+
+//! The memory of a solve in a matrix's workspace.
+static gm_memory memory_of(akm_gmres_workspace &ws)
+{
+  gm_memory mem;
+  mem.V = ws.V.data();
+  mem.work = ws.work.data();
+  mem.state = ws.state.data();
+  mem.host_state = ws.host_state;
+  mem.ready = ws.state_ready;
+  return mem;
+}
+
 //! One restart cycle's device state and the kernel sequence of its steps.
 template<typename P>
 struct gm_cycle
@@ -530,26 +628,26 @@ struct gm_cycle
   hipEvent_t *ready = nullptr;
   int rotate_always = 0;
 
-  gm_cycle(akm_gmres_workspace &ws, int64_t n_, hipStream_t s_, int restart_, P *Z_,
+  gm_cycle(gm_memory const &mem, int64_t n_, hipStream_t s_, int restart_, P *Z_,
            int rotate_always_)
       : n(n_), s(s_), restart(restart_), Z(Z_), rotate_always(rotate_always_)
   {
-    V = reinterpret_cast<P *>(ws.V.data());
+    V = static_cast<P *>(mem.V);
     ldv = n;
     rpb = rows_per_block_for(n);
     nblk = static_cast<int>((n + rpb - 1) / rpb);
     sblocks = static_cast<int>(std::min<int64_t>((n + gm_threads - 1) / gm_threads, 2048));
     // small work arrays: h, H (packed), cs, sn, g, dot partials, sq partials
-    h = reinterpret_cast<P *>(ws.work.data());
+    h = static_cast<P *>(mem.work);
     H = h + (restart + 1);
     cs = H + int64_t(restart) * (restart + 1) / 2 + 1;
     sn = cs + (restart + 1);
     g = sn + (restart + 1);
     dotp = g + (restart + 2);
     sqp = dotp + int64_t(restart + 1) * nblk;
-    st = reinterpret_cast<gm_state<P> *>(ws.state.data());
-    hst = static_cast<gm_state<P> *>(ws.host_state);
-    ready = ws.state_ready;
+    st = static_cast<gm_state<P> *>(mem.state);
+    hst = static_cast<gm_state<P> *>(mem.host_state);
+    ready = mem.ready;
   }
 
   int sync_state()
@@ -615,7 +713,8 @@ struct gm_cycle
     AKM_TRY(check_launch("gmres inner iteration"));
     AKM_HIP(hipMemcpyAsync(hst + (j % gm_slots), st, sizeof(gm_state<P>),
                            hipMemcpyDeviceToHost, s));
-    AKM_HIP(hipEventRecord(ready[j % gm_slots], s));
+    if (ready != nullptr)
+      AKM_HIP(hipEventRecord(ready[j % gm_slots], s));
     return AKM_SUCCESS;
   }
 
@@ -647,7 +746,10 @@ struct gm_cycle
   //! Waits for step j's state; est = its residual estimate.
   int wait_step(int j, P &est)
   {
-    AKM_HIP(hipEventSynchronize(ready[j % gm_slots]));
+    if (ready != nullptr)
+      AKM_HIP(hipEventSynchronize(ready[j % gm_slots]));
+    else
+      AKM_HIP(hipStreamSynchronize(s));
     est = hst[j % gm_slots].inner_res;
     return AKM_SUCCESS;
   }
@@ -665,13 +767,48 @@ struct gm_cycle
   }
 };
 
+//! The restart loop of simple_gmres (src/solver.cpp:201-295) on a cycle:
+//! `residual(outer, speculate)` writes b - op(x) (preconditioned, for a left
+//! preconditioner) into V_0 and may set `speculate`; then the Arnoldi steps,
+//! the update of x and the absolute test of the residual estimate.
+template<typename P, typename R>
+static int gmres_outer(gm_cycle<P> &cyc, R &&residual,
+                       std::function<int(const P *, P *)> const &op, P *x, int restart,
+                       int max_outer, P tol, double *err, int *iterations)
+{
+  int total = 0, outer = 0;
+  P inner_res = 0, outer_res = tol + 1;
+  bool speculate = false;
+  while (outer_res > tol && outer < max_outer)
+  {
+    AKM_TRY(residual(outer, speculate));
+    ++total;
+    AKM_TRY(cyc.start(tol, inner_res));
+    int inner = 0;
+    AKM_TRY(cyc.run_steps(restart, op, nullptr, tol, tol, inner_res, inner, speculate));
+    total += inner;
+    AKM_TRY(cyc.finish(inner, x, false));
+    ++outer;
+    outer_res = inner_res;
+  }
+  AKM_HIP(hipStreamSynchronize(cyc.s));
+  if (err)
+    *err = static_cast<double>(outer_res);
+  if (iterations)
+    *iterations = total;
+  return (outer_res > tol) ? AKM_ERR_NOT_CONVERGED : AKM_SUCCESS;
+}
+
+//! The solve of akm_gmres_euler on device vectors in the memory `mem`.
+//! Without events in `mem` the host waits for the whole stream after every
+//! step, and no step is enqueued ahead.
 template<typename P>
-static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int max_outer,
-                       P tol, double *err, int *iterations)
+static int gmres_typed(akm_matrix_s &m, gm_memory const &mem, P dt, P *x, const P *b,
+                       int restart, int max_outer, P tol, double *err, int *iterations)
 {
   const int64_t n = m.output_size();
   hipStream_t s = m.stream;
-  gm_cycle<P> cyc(*m.gmres, n, s, restart, nullptr, 0);
+  gm_cycle<P> cyc(mem, n, s, restart, nullptr, 0);
   P *V = cyc.V;
   // the global operator's solve is left-preconditioned by its Jacobi
   // diagonal, fused into the apply (src/solver.cpp:94-118)
@@ -684,40 +821,21 @@ static int gmres_typed(akm_matrix_s &m, P dt, P *x, const P *b, int restart, int
   const std::function<int(const P *, P *)> op = [&](const P *v, P *w) -> int {
     return apply_op(-dt, v, P{0}, w, P{1}, v);
   };
-
-  int total = 0, outer = 0;
-  P inner_res = 0, outer_res = tol + 1;
-  bool speculate = false;
-  while (outer_res > tol && outer < max_outer)
-  {
-    // r = b - (I - dt A) x into V_0
+  // r = b - (I - dt A) x into V_0; the first one is timed (when events are
+  // at hand) to decide whether steps are enqueued ahead
+  auto residual = [&](int outer, bool &speculate) -> int {
     AKM_HIP(hipMemcpyAsync(V, b, sizeof(P) * n, hipMemcpyDeviceToDevice, s));
-    if (outer == 0)
+    if (outer == 0 && mem.ready != nullptr)
     {
       double ms = 0;
       AKM_TRY(timed_on_stream(s, [&]() { return apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x); },
                               ms));
       speculate = ms < speculate_below_ms;
+      return AKM_SUCCESS;
     }
-    else
-    {
-      AKM_TRY(apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x));
-    }
-    ++total;
-    AKM_TRY(cyc.start(tol, inner_res));
-    int inner = 0;
-    AKM_TRY(cyc.run_steps(restart, op, nullptr, tol, tol, inner_res, inner, speculate));
-    total += inner;
-    AKM_TRY(cyc.finish(inner, x, false));
-    ++outer;
-    outer_res = inner_res;
-  }
-  AKM_HIP(hipStreamSynchronize(s));
-  if (err)
-    *err = static_cast<double>(outer_res);
-  if (iterations)
-    *iterations = total;
-  return (outer_res > tol) ? AKM_ERR_NOT_CONVERGED : AKM_SUCCESS;
+    return apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x);
+  };
+  return gmres_outer<P>(cyc, residual, op, x, restart, max_outer, tol, err, iterations);
 }
 
 //! The mixed-precision / flexible solver (FP64 matrix m64).
@@ -729,7 +847,7 @@ static int gmres_mixed(akm_matrix_s &m64, akm_matrix_s *m32, akm_precond_s *pre,
   hipStream_t s = m64.stream;
   auto &ws = *m64.gmres;
   double *Z = (pre != nullptr) ? ws.Z.data() : nullptr;
-  gm_cycle<double> cyc(ws, n, s, restart, Z, 1);
+  gm_cycle<double> cyc(memory_of(ws), n, s, restart, Z, 1);
   double *V = cyc.V;
   const int sblocks = cyc.sblocks;
 
@@ -843,26 +961,19 @@ static int default_restarts(int64_t n, size_t esize)
   return std::clamp(guess, minimum, maximum);
 }
 
-static int ensure_workspace(akm_matrix_s &m, int restart)
+//! Sizes the workspace for n unknowns, `restart` and elements of `esize`
+//! bytes (no work when it already has that shape).
+static int size_workspace(akm_gmres_workspace &ws, int64_t n, int restart, size_t esize)
 {
-  const int64_t n = m.output_size();
-  if (!m.gmres)
-    m.gmres = std::unique_ptr<akm_gmres_workspace, void (*)(akm_gmres_workspace *)>(
-        new akm_gmres_workspace(), delete_workspace);
-  auto &ws = *m.gmres;
-  if (ws.restart == restart && ws.n == n && ws.esize == m.esize)
+  if (ws.restart == restart && ws.n == n && ws.esize == esize)
     return AKM_SUCCESS;
   ws.restart = restart;
   ws.n = n;
-  ws.esize = m.esize;
-  AKM_TRY(ws.V.allocate(n * (restart + 1) * int64_t(m.esize)));
-  const int64_t rpb = rows_per_block_for(n);
-  const int64_t nblk = (n + rpb - 1) / rpb;
-  const int64_t small = 4 * int64_t(restart + 2) + int64_t(restart) * (restart + 1) / 2 + 2 +
-                        int64_t(restart + 1) * nblk + nblk + 16;
-  AKM_TRY(ws.work.allocate(small * int64_t(m.esize)));
+  ws.esize = esize;
+  AKM_TRY(ws.V.allocate(n * (restart + 1) * int64_t(esize)));
+  AKM_TRY(ws.work.allocate(gm_work_values(n, restart) * int64_t(esize)));
   AKM_TRY(ws.state.allocate(sizeof(gm_state<double>)));
-  AKM_TRY(ws.xb.allocate(2 * n * int64_t(m.esize)));
+  AKM_TRY(ws.xb.allocate(2 * n * int64_t(esize)));
   if (ws.host_state == nullptr)
     AKM_HIP(hipHostMalloc(&ws.host_state, gm_slots * sizeof(gm_state<double>),
                           hipHostMallocDefault));
@@ -876,6 +987,104 @@ static int ensure_workspace(akm_matrix_s &m, int restart)
   ws.y32.reset();
   return AKM_SUCCESS;
 }
+
+static int ensure_workspace(akm_matrix_s &m, int restart)
+{
+  if (!m.gmres)
+    m.gmres = std::unique_ptr<akm_gmres_workspace, void (*)(akm_gmres_workspace *)>(
+        new akm_gmres_workspace(), delete_workspace);
+  return size_workspace(*m.gmres, m.output_size(), restart, m.esize);
+}
+
+//! LU factorisation with partial pivoting of the n x n column-major a, in
+//! place (getrf): piv[k] is the row swapped with row k at step k.  Returns
+//! false for an exactly singular matrix.
+template<typename P>
+static bool lu_factor(int n, std::vector<P> &a, std::vector<int> &piv)
+{
+  piv.assign(n, 0);
+  for (int k = 0; k < n; k++)
+  {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::abs(a[int64_t(k) * n + i]) > std::abs(a[int64_t(k) * n + p]))
+        p = i;
+    piv[k] = p;
+    if (a[int64_t(k) * n + p] == P{0})
+      return false;
+    if (p != k)
+      for (int j = 0; j < n; j++)
+        std::swap(a[int64_t(j) * n + k], a[int64_t(j) * n + p]);
+    const P inv = P{1} / a[int64_t(k) * n + k];
+    for (int i = k + 1; i < n; i++)
+      a[int64_t(k) * n + i] *= inv;
+    for (int j = k + 1; j < n; j++)
+    {
+      const P akj = a[int64_t(j) * n + k];
+      if (akj != P{0})
+        for (int i = k + 1; i < n; i++)
+          a[int64_t(j) * n + i] -= a[int64_t(k) * n + i] * akj;
+    }
+  }
+  return true;
+}
+
+//! simple_gmres on an explicit matrix (src/solver.cpp:54-74): host A, x, b
+//! and optional M; the solve on the null stream with its own workspace.
+template<typename P>
+static int gmres_dense_typed(int n, const P *A_h, P *x_h, const P *b_h, const P *M_h,
+                             int restart, int max_outer, P tol, double *err, int *iterations)
+{
+  constexpr int max_lu = 4096; // v in one workgroup's LDS
+  if (M_h != nullptr && n > max_lu)
+    return set_error(AKM_ERR_UNSUPPORTED, "gmres dense: preconditioner of order %d above %d", n,
+                     max_lu);
+  hipStream_t s = nullptr;
+  dev_buffer<P> A, xb, LU;
+  dev_buffer<int> piv;
+  AKM_TRY(A.upload(A_h, int64_t(n) * n, s));
+  AKM_TRY(xb.allocate(2 * int64_t(n)));
+  P *x = xb.data();
+  P *b = xb.data() + n;
+  AKM_HIP(hipMemcpyAsync(x, x_h, sizeof(P) * n, hipMemcpyHostToDevice, s));
+  AKM_HIP(hipMemcpyAsync(b, b_h, sizeof(P) * n, hipMemcpyHostToDevice, s));
+  if (M_h != nullptr)
+  {
+    std::vector<P> lu(M_h, M_h + int64_t(n) * n);
+    std::vector<int> p;
+    if (!lu_factor<P>(n, lu, p))
+      return set_error(AKM_ERR_INVALID, "gmres dense: the preconditioner is singular");
+    AKM_TRY(LU.upload(lu.data(), int64_t(n) * n, s));
+    AKM_TRY(piv.upload(p.data(), n, s));
+  }
+  akm_gmres_workspace ws;
+  AKM_TRY(size_workspace(ws, n, restart, sizeof(P)));
+  gm_cycle<P> cyc(memory_of(ws), n, s, restart, nullptr, 0);
+  const int gblocks = (n + gm_threads - 1) / gm_threads;
+  auto precondition = [&](P *v) -> int {
+    if (M_h == nullptr)
+      return AKM_SUCCESS;
+    gm_lu_solve<P><<<1, 256, sizeof(P) * n, s>>>(n, LU.data(), piv.data(), v);
+    return check_launch("gmres dense preconditioner");
+  };
+  const std::function<int(const P *, P *)> op = [&](const P *v, P *w) -> int {
+    gm_dense_gemv<P><<<gblocks, gm_threads, 0, s>>>(n, A.data(), v, P{1}, P{0}, w);
+    AKM_TRY(check_launch("gmres dense gemv"));
+    return precondition(w);
+  };
+  auto residual = [&](int, bool &) -> int {
+    AKM_HIP(hipMemcpyAsync(cyc.V, b, sizeof(P) * n, hipMemcpyDeviceToDevice, s));
+    gm_dense_gemv<P><<<gblocks, gm_threads, 0, s>>>(n, A.data(), x, P{-1}, P{1}, cyc.V);
+    AKM_TRY(check_launch("gmres dense gemv"));
+    return precondition(cyc.V);
+  };
+  const int rc = gmres_outer<P>(cyc, residual, op, x, restart, max_outer, tol, err, iterations);
+  if (rc < 0)
+    return rc;
+  AKM_HIP(hipMemcpy(x_h, x, sizeof(P) * n, hipMemcpyDeviceToHost));
+  return rc;
+}
+
 
 } // namespace akm
 
@@ -917,11 +1126,13 @@ extern "C" akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const 
                            mat->stream));
   }
   int rc;
+  const gm_memory mem = memory_of(ws);
   if (mat->prec == AKM_F64)
-    rc = gmres_typed<double>(*mat, dt, static_cast<double *>(xd), static_cast<const double *>(bd),
-                             restart, max_iter, tol, err, iterations);
+    rc = gmres_typed<double>(*mat, mem, dt, static_cast<double *>(xd),
+                             static_cast<const double *>(bd), restart, max_iter, tol, err,
+                             iterations);
   else
-    rc = gmres_typed<float>(*mat, static_cast<float>(dt), static_cast<float *>(xd),
+    rc = gmres_typed<float>(*mat, mem, static_cast<float>(dt), static_cast<float *>(xd),
                             static_cast<const float *>(bd), restart, max_iter,
                             static_cast<float>(tol), err, iterations);
   if (rc < 0)
@@ -991,4 +1202,36 @@ extern "C" akm_status akm_gmres_euler_mixed(akm_matrix mat, akm_matrix mat32, ak
   if (stats != nullptr)
     *stats = local;
   return rc;
+}
+
+extern "C" akm_status akm_gmres_dense(int n, const void *A, void *x, const void *b,
+                                      const void *M, int restart, int max_iter, double tol,
+                                      akm_prec prec, double *err, int *iterations)
+{
+  if (A == nullptr || x == nullptr || b == nullptr)
+    return set_error(AKM_ERR_INVALID, "gmres dense: null argument");
+  if (n <= 0)
+    return set_error(AKM_ERR_INVALID, "gmres dense: empty system");
+  if (prec != AKM_F64 && prec != AKM_F32)
+    return set_error(AKM_ERR_INVALID, "gmres dense: unknown precision %d", int(prec));
+  const size_t esize = (prec == AKM_F64) ? sizeof(double) : sizeof(float);
+  if (tol <= 0)
+    tol = (prec == AKM_F64) ? 1e-12 : 1e-6;
+  if (restart <= 0)
+    restart = default_restarts(n, esize);
+  if (restart > n)
+    return set_error(AKM_ERR_INVALID, "Number of inner iterations %d must be less than %d!",
+                     restart, n);
+  if (restart > gm_max_restart)
+    return set_error(AKM_ERR_UNSUPPORTED, "gmres: restart %d above %d", restart, gm_max_restart);
+  if (max_iter <= 0)
+    max_iter = n;
+  if (prec == AKM_F64)
+    return gmres_dense_typed<double>(n, static_cast<const double *>(A), static_cast<double *>(x),
+                                     static_cast<const double *>(b),
+                                     static_cast<const double *>(M), restart, max_iter, tol,
+                                     err, iterations);
+  return gmres_dense_typed<float>(n, static_cast<const float *>(A), static_cast<float *>(x),
+                                  static_cast<const float *>(b), static_cast<const float *>(M),
+                                  restart, max_iter, static_cast<float>(tol), err, iterations);
 }

@@ -235,6 +235,22 @@ akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, in
                            int max_iter, double tol, int on_device, double *err,
                            int *iterations);
 
+/* solver::simple_gmres(fk::matrix A, fk::vector x, fk::vector b, fk::matrix M,
+ * restart, max_iter, tol) (src/solver.hpp:17-22, src/solver.cpp:54-74): the
+ * reference's node-local GMRES on an explicit n x n matrix, with the same
+ * algorithm, defaults, absolute tolerance and error behaviour as
+ * akm_gmres_euler.  A and M are column-major host arrays (fk::matrix layout),
+ * x (initial guess in, solution out) and b host vectors, all of precision
+ * prec.  M (NULL = none) is a left preconditioner applied as M^-1 through its
+ * LU factors with partial pivoting, as dense_preconditioner does with
+ * getrf/getrs (src/solver.cpp:11-38), to the residual and to every new
+ * Krylov vector; n <= 4096 with M.  A singular M returns AKM_ERR_INVALID.
+ * The products and solves run on the GPU (null stream); the call returns
+ * with x on the host. */
+akm_status akm_gmres_dense(int n, const void *A, void *x, const void *b, const void *M,
+                           int restart, int max_iter, double tol, akm_prec prec, double *err,
+                           int *iterations);
+
 /* --------------------------------------------------- DG operators (1D) -- */
 
 #define AKM_DG_MASS 0    /* int g phi_j psi_i */

@@ -112,3 +112,53 @@ def test_gmres_fp32():
     assert info.error <= 1e-5
     gold = oracle.dense_solve(np.eye(size) - dt * dense_operator(c), b)
     assert np.abs(host(x) - gold).max() < 1e-4
+
+
+# ---- solver::simple_gmres on an explicit matrix (akm_gmres_dense) ---------
+
+A_GOLD = np.array([[3.383861628748717e+00, 1.113343240310116e-02, 2.920740795411032e+00],
+                   [3.210305545769361e+00, 3.412141162288144e+00, 3.934494120167269e+00],
+                   [1.723479266939425e+00, 1.710451084172946e+00, 4.450671104482062e+00]])
+PRECOND = np.diag([3.383861628748717e+00, 3.412141162288144e+00, 4.450671104482062e+00])
+B_GOLD = np.array([2.084406360034887e-01, 6.444769305362776e-01, 3.687335330031937e-01])
+X_GOLD = np.array([4.715561567725287e-02, 1.257695999382253e-01, 1.625351700791827e-02])
+B_GOLD_2 = np.array([9.789303188021963e-01, 8.085725142873675e-01, 7.370498473207234e-01])
+X_GOLD_2 = np.array([1.812300946484165e-01, -7.824949213916167e-02, 1.254969087137521e-01])
+
+
+@pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+@pytest.mark.parametrize("b,xg", [(B_GOLD, X_GOLD), (B_GOLD_2, X_GOLD_2)])
+@pytest.mark.parametrize("M", [None, PRECOND])
+def test_dense_gmres_3x3_gold(dtype, rtol, b, xg, M):
+    """The reference's gold (src/solver_tests.cpp:141-213): restart = max
+    outer = 3, tolerance = machine epsilon, with and without the point
+    Jacobi preconditioner."""
+    eps = float(np.finfo(dtype).eps)
+    x = np.zeros(3, dtype)
+    info = asgard_amd.simple_gmres(A_GOLD.astype(dtype), x, b.astype(dtype),
+                                   None if M is None else M.astype(dtype), 3, 3, eps)
+    assert info.error < eps
+    assert np.allclose(x, xg, rtol=rtol, atol=10 * eps)
+
+
+def test_dense_gmres_matches_oracle():
+    n = 120
+    rng = np.random.default_rng(11)
+    A = np.eye(n) * 4 + rng.uniform(-1, 1, (n, n)) / np.sqrt(n)
+    M = np.diag(np.diag(A)) + np.tril(rng.uniform(-0.1, 0.1, (n, n)), -1)
+    b = rng.uniform(-1, 1, n)
+    for pre in (None, M):
+        xo, erro, itso = oracle.gmres_dense(A, b, M=pre, restart=20, max_outer=50, tol=1e-12)
+        x = np.zeros(n)
+        info = asgard_amd.simple_gmres(A, x, b, pre, 20, 50, 1e-12)
+        assert info.error <= 1e-12
+        assert abs(info.iterations - itso) <= 1
+        assert np.abs(x - xo).max() <= 1e-10
+        assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-9
+
+
+def test_dense_gmres_errors():
+    with pytest.raises(ValueError):
+        asgard_amd.simple_gmres(A_GOLD, np.zeros(3), B_GOLD, None, 4, 3, 1e-12)
+    with pytest.raises(ValueError):
+        asgard_amd.simple_gmres(A_GOLD, np.zeros(3), B_GOLD, np.zeros((3, 3)), 3, 3, 1e-12)
