@@ -48,7 +48,8 @@ EXPORTS = (
     "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
     "akm_update_operators", "akm_imex_step", "akm_dg_term",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
-    "akm_autotune", "akm_time_apply", "akm_destroy", "akm_gmres_euler", "akm_gmres_dense",
+    "akm_autotune", "akm_set_kernel_family", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
+    "akm_gmres_dense",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
     "akm_mlp_get_params", "akm_mlp_set_params", "akm_mlp_forward", "akm_mlp_gradients",
     "akm_mlp_adam_step", "akm_mlp_train", "akm_precond_apply", "akm_precond_destroy",
@@ -142,6 +143,7 @@ def load_library(path: str = LIB_PATH):
     L.akm_set_tuning.argtypes = [_vp, _i, _i, _i]
     L.akm_time_apply.argtypes = [_vp, _d, _vp, _d, _vp, _i, ctypes.POINTER(_d)]
     L.akm_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]
+    L.akm_set_kernel_family.argtypes = [_vp, _i]
     L.akm_destroy.argtypes = [_vp]
     L.akm_gmres_euler.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
                                   ctypes.POINTER(_i)]
@@ -414,16 +416,22 @@ class KronmultMatrix:
                                               py))
         return y
 
+    def set_kernel_family(self, family):
+        """0 = register / LDS-table kernels, 1 = sum-factorised matrix-core
+        kernel (FP64 sparse-grid, d = 3, n = 3)."""
+        _check(load_library().akm_set_kernel_family(self._h, int(family)))
+
     def autotune(self, x, y):
         """Times the kernel shapes on this operator (y is overwritten) and
-        keeps the fastest; returns (waves_per_row, lds_tables)."""
+        keeps the fastest; returns (waves_per_row, kernel) with kernel 0 =
+        factors from global memory, 1 = LDS factor tables, 2 = matrix cores."""
         px, dx = _vec_ptr(x, self.dtype)
         py, dy = _vec_ptr(y, self.dtype)
         if not (dx and dy):
             raise AkmInvalid(AKM_ERR_INVALID, "autotune needs device tensors")
         w, l = _i(), _i()
         _check(load_library().akm_autotune(self._h, px, py, ctypes.byref(w), ctypes.byref(l)))
-        return w.value, bool(l.value)
+        return w.value, l.value
 
     def time_apply(self, alpha, x, beta, y, iterations):
         px, dx = _vec_ptr(x, self.dtype)

@@ -115,6 +115,9 @@ int launch_kronmult(int D, int n, int mode, kron_args<P> const &a, int num_rows,
                     int force_generic, hipStream_t stream);
 template<typename P>
 int launch_scale(int64_t size, P beta, P *y, P gamma, const P *xg, hipStream_t stream);
+//! True when the sum-factorised matrix-core kernel (kernel family 1) can run
+//! an FP64 operator of this shape; `packed` = packed cell words present.
+bool mfma_family_applies(int D, int n, int mode, int T, bool packed);
 
 struct batch_system;
 //! Batched sparse-grid FP64 kronmult over a (system, row) work list of

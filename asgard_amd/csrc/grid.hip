@@ -138,11 +138,12 @@ __device__ __forceinline__ bool connected_cells(const int *__restrict__ rc,
 
 //! One wavefront per row; lanes test 64 columns at a time.  In the count
 //! pass `indx` is null and the row length goes to counts[row]; in the fill
-//! pass the connected columns are written in ascending order at pntr[row].
+//! pass the connected columns are written at pntr[row] in the order of the
+//! candidate list: ascending cell ids, or order[0..num_cells) when given.
 template<int D>
 __global__ void __launch_bounds__(256) pattern_scan(int64_t num_cells, const int *table,
                                                     int64_t *counts, const int64_t *pntr,
-                                                    int *indx)
+                                                    int *indx, const int *order)
 {
   const int lane = threadIdx.x & 63;
   const int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -156,9 +157,10 @@ __global__ void __launch_bounds__(256) pattern_scan(int64_t num_cells, const int
   int *out = (indx != nullptr) ? indx + pntr[row] : nullptr;
   for (int64_t c0 = 0; c0 < num_cells; c0 += 64)
   {
-    const int64_t col = c0 + lane;
+    const int64_t pos = c0 + lane;
+    const int64_t col = (order != nullptr && pos < num_cells) ? order[pos] : pos;
     bool conn = false;
-    if (col < num_cells)
+    if (pos < num_cells)
     {
       int cc[2 * D];
 #pragma unroll
@@ -180,41 +182,61 @@ __global__ void __launch_bounds__(256) pattern_scan(int64_t num_cells, const int
 
 template<int D>
 static int scan_pass(int64_t num_cells, const int *table_dev, int64_t *counts,
-                     const int64_t *pntr, int *indx)
+                     const int64_t *pntr, int *indx, const int *order)
 {
   const int64_t blocks = (num_cells + 3) / 4;
   pattern_scan<D><<<dim3(static_cast<unsigned>(blocks)), 256>>>(num_cells, table_dev, counts,
-                                                                  pntr, indx);
+                                                                  pntr, indx, order);
   return check_launch("sparse pattern scan");
 }
 
 static int scan_dispatch(int d, int64_t num_cells, const int *table_dev, int64_t *counts,
-                         const int64_t *pntr, int *indx)
+                         const int64_t *pntr, int *indx, const int *order)
 {
   switch (d)
   {
-  case 1: return scan_pass<1>(num_cells, table_dev, counts, pntr, indx);
-  case 2: return scan_pass<2>(num_cells, table_dev, counts, pntr, indx);
-  case 3: return scan_pass<3>(num_cells, table_dev, counts, pntr, indx);
-  case 4: return scan_pass<4>(num_cells, table_dev, counts, pntr, indx);
-  case 5: return scan_pass<5>(num_cells, table_dev, counts, pntr, indx);
-  case 6: return scan_pass<6>(num_cells, table_dev, counts, pntr, indx);
+  case 1: return scan_pass<1>(num_cells, table_dev, counts, pntr, indx, order);
+  case 2: return scan_pass<2>(num_cells, table_dev, counts, pntr, indx, order);
+  case 3: return scan_pass<3>(num_cells, table_dev, counts, pntr, indx, order);
+  case 4: return scan_pass<4>(num_cells, table_dev, counts, pntr, indx, order);
+  case 5: return scan_pass<5>(num_cells, table_dev, counts, pntr, indx, order);
+  case 6: return scan_pass<6>(num_cells, table_dev, counts, pntr, indx, order);
   default: return set_error(AKM_ERR_UNSUPPORTED, "sparse pattern: d = %d not in [1, 6]", d);
   }
 }
 
+void lexicographic_order(int d, int64_t num_cells, const int *elem, std::vector<int> &order)
+{
+  order.resize(num_cells);
+  for (int64_t c = 0; c < num_cells; c++)
+    order[c] = static_cast<int>(c);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    for (int k = 0; k < d; k++)
+    {
+      const int ea = elem[int64_t(a) * d + k], eb = elem[int64_t(b) * d + k];
+      if (ea != eb)
+        return ea < eb;
+    }
+    return false;
+  });
+}
+
 int device_sparse_pattern(int d, int64_t num_cells, const int *table_host,
-                          std::vector<int64_t> &pntr, dev_buffer<int> &indx_dev)
+                          std::vector<int64_t> &pntr, dev_buffer<int> &indx_dev,
+                          const int *order_host)
 {
   if (num_cells <= 0)
     return set_error(AKM_ERR_INVALID, "sparse pattern: no cells");
   if (num_cells > (int64_t(1) << 31) - 2)
     return set_error(AKM_ERR_INVALID, "sparse pattern: too many cells");
-  dev_buffer<int> table_dev;
+  dev_buffer<int> table_dev, order_dev;
   AKM_TRY(table_dev.upload(table_host, num_cells * 2 * d));
+  if (order_host != nullptr)
+    AKM_TRY(order_dev.upload(order_host, num_cells));
   dev_buffer<int64_t> counts, pntr_dev;
   AKM_TRY(counts.allocate(num_cells));
-  AKM_TRY(scan_dispatch(d, num_cells, table_dev.data(), counts.data(), nullptr, nullptr));
+  AKM_TRY(scan_dispatch(d, num_cells, table_dev.data(), counts.data(), nullptr, nullptr,
+                        nullptr));
   std::vector<int64_t> cnt(num_cells);
   AKM_HIP(hipMemcpy(cnt.data(), counts.data(), sizeof(int64_t) * num_cells,
                     hipMemcpyDeviceToHost));
@@ -224,7 +246,7 @@ int device_sparse_pattern(int d, int64_t num_cells, const int *table_host,
   AKM_TRY(pntr_dev.upload(pntr.data(), num_cells + 1));
   AKM_TRY(indx_dev.allocate(pntr[num_cells]));
   AKM_TRY(scan_dispatch(d, num_cells, table_dev.data(), nullptr, pntr_dev.data(),
-                        indx_dev.data()));
+                        indx_dev.data(), order_dev.data()));
   AKM_HIP(hipDeviceSynchronize());
   return AKM_SUCCESS;
 }

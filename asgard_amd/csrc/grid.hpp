@@ -21,7 +21,14 @@ int build_offset_table(int max_level, std::vector<int> &table);
 //! 2^(level-1) + position).
 void cell_1d_indices(int d, int64_t num_cells, const int *table, std::vector<int> &elem);
 
-//! Cell-pair CSR on the GPU: pntr (host) and the column ids (device).
+//! Cell-pair CSR on the GPU: pntr (host) and the column ids (device), each
+//! row's columns ascending, or in the order of order_host (a permutation of
+//! the cells) when it is not null.
 int device_sparse_pattern(int d, int64_t num_cells, const int *table_host,
-                          std::vector<int64_t> &pntr, dev_buffer<int> &indx_dev);
+                          std::vector<int64_t> &pntr, dev_buffer<int> &indx_dev,
+                          const int *order_host = nullptr);
+
+//! The cells sorted lexicographically by their 1D indices (dimension 0
+//! first), ties by cell id.
+void lexicographic_order(int d, int64_t num_cells, const int *elem, std::vector<int> &order);
 } // namespace akm

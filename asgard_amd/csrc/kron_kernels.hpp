@@ -78,6 +78,9 @@ struct kron_args
   int lds_cap;                      // LDS bytes a workgroup may use (0: no tables)
   // rows in dispatch order (longest first)
   const int *row_order;
+  // kernel family: 0 register / LDS-table kernels, 1 the sum-factorised
+  // matrix-core kernel (FP64, sparse-grid, d = 3, n = 3)
+  int family;
   // vectors
   const P *x;
   P *y;

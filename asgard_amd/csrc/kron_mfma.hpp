@@ -1,0 +1,653 @@
+// Sum-factorised kronmult on the FP64 matrix cores: d = 3, n = 3, sparse-grid
+// mode (included by kron_f64.hip).
+//
+// For an output row r and a term t the apply is
+//   y_r += sum over connected columns c of (A0[r0,c0] x A1[r1,c1] x A2[r2,c2]) x_c
+// and the factor of dimension k depends only on (r_k, c_k).  With the row's
+// columns in lexicographic order of their 1D indices (c0, c1, c2) the sum
+// factorises: dimension 2 is contracted column by column and summed over the
+// columns that share (c0, c1); that sum is contracted over dimension 1 and
+// summed over the groups that share c0; that sum is contracted over
+// dimension 0 once per c0 group.  Per (pair, term) the first contraction is
+// the only per-column work; the other two run once per group.
+//
+// One wavefront holds one 3 x 3 x 3 tensor, padded to 4 x 4 x 4, in the lane
+// layout of v_mfma_f64_4x4x4_4b_f64: with lane = 16 q + 4 g + m the
+// instruction runs four independent 4 x 4 x 4 products (block g) with
+// A[i][k] in lane (i = m, k = q), B[k][j] in lane (k = q, j = m) and the
+// result D[i][j] in lane (i = q, j = m).  The result therefore sits where the
+// next product's B operand must be, so the dimension-2 and dimension-1
+// contractions chain on the matrix cores without moving data, and the
+// matrix cores' accumulator input does the group sums:
+//   stage 1 (dim 2): block g = i0, A = x[i0][i1 = m][i2 = q],
+//                    B = A2(j2 = m, i2 = q)          -> D1[i1 = q][j2 = m]
+//   stage 2 (dim 1): A = A1(j1 = m, i1 = q), B = D1  -> D2[j1 = q][j2 = m]
+//   stage 3 (dim 0): lane (j1 = q, j0 = g, j2 = m) sums A0(g, i0) D2 over the
+//                    blocks i0 = 0..2 (lane reads within the 16-lane row).
+// Padding entries (index 3) hold zeros in every operand.
+//
+// A workgroup of W wavefronts owns a row; wavefront w takes the w-th of W
+// contiguous pieces of the row's column list (each piece closes its own
+// groups, which is exact because the apply is linear in the columns).  The
+// pieces' sums meet in LDS in wavefront order: results repeat bit for bit.
+#pragma once
+
+#include "kron_impl.hpp"
+
+namespace akm
+{
+constexpr int mfma_max_terms = 8;
+
+__device__ __forceinline__ double mfma4(double a, double b, double c)
+{
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+//! 1D index of dimension k from a packed cell word (10 bits per dimension).
+__device__ __forceinline__ int packed_1d(unsigned long long w, int k)
+{
+  return static_cast<int>((w >> (10 * k)) & 1023u);
+}
+
+template<int W, int TT>
+__global__ void __launch_bounds__(W *wave_size) kron_mfma_d3n3(kron_args<double> a)
+{
+  constexpr int N = 27;
+  __shared__ double red[W * wave_size];
+  const int lane = threadIdx.x & (wave_size - 1);
+  const int wave = threadIdx.x / wave_size;
+  const int q = lane >> 4, g = (lane >> 2) & 3, m = lane & 3;
+  const int T = a.T;
+
+  const int row = a.row_order[blockIdx.x];
+  const int64_t jbeg = a.pntr[row], jend = a.pntr[row + 1];
+  const int64_t len = jend - jbeg;
+  const int64_t cb = jbeg + (len * wave) / W, ce = jbeg + (len * (wave + 1)) / W;
+
+  const int r0M = a.elem[int64_t(row) * 3 + 0] * a.M;
+  const int r1M = a.elem[int64_t(row) * 3 + 1] * a.M;
+  const int r2M = a.elem[int64_t(row) * 3 + 2] * a.M;
+  const int64_t blk = int64_t(T) * 27; // one connection: T terms x 3 dims x 9
+
+  // lane roles: the x entry of the stage-1 A operand, the factor entry of
+  // the stage-1 B and stage-2 A operands (both F[q*3 + m]), the stage-3 rows
+  const bool xvalid = (g < 3 && m < 3 && q < 3);
+  const int xoff = g * 9 + m * 3 + q;
+  const bool fvalid = (q < 3 && m < 3);
+  const int foff = q * 3 + m;
+
+  double s1[TT], s2[TT];
+  double f1[TT];    // stage-2 factor of the open (c0, c1) group
+  double f0[TT][3]; // stage-3 factor A0(g, i0) of the open c0 group
+#pragma unroll
+  for (int t = 0; t < TT; t++)
+  {
+    s1[t] = 0;
+    s2[t] = 0;
+    f1[t] = 0;
+    f0[t][0] = f0[t][1] = f0[t][2] = 0;
+  }
+  double acc = 0;
+
+  auto load_f1 = [&](int c1) {
+    const int64_t base = int64_t(a.off1d[r1M + c1]) * blk + 9;
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+        f1[t] = fvalid ? a.vA[base + t * 27 + foff] : 0.0;
+  };
+  auto load_f0 = [&](int c0) {
+    const int64_t base = int64_t(a.off1d[r0M + c0]) * blk;
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+#pragma unroll
+        for (int i0 = 0; i0 < 3; i0++)
+          f0[t][i0] = (g < 3) ? a.vA[base + t * 27 + i0 * 3 + g] : 0.0;
+  };
+  // closes the open (c0, c1) group: its dimension-2 sum is contracted over
+  // dimension 1 into the c0 group's sum
+  auto close_01 = [&]() {
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+      {
+        s2[t] = mfma4(f1[t], s1[t], s2[t]);
+        s1[t] = 0;
+      }
+  };
+  // closes the open c0 group: contraction over dimension 0 into acc
+  auto close_0 = [&]() {
+    const int src = 16 * q + m;
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+      {
+        const double v0 = __shfl(s2[t], src, wave_size);
+        const double v1 = __shfl(s2[t], src + 4, wave_size);
+        const double v2 = __shfl(s2[t], src + 8, wave_size);
+        acc = fma(f0[t][0], v0, acc);
+        acc = fma(f0[t][1], v1, acc);
+        acc = fma(f0[t][2], v2, acc);
+        s2[t] = 0;
+      }
+  };
+
+  unsigned int prev01 = 0xFFFFFFFFu, prev0 = 0xFFFFFFFFu;
+  for (int64_t j0 = cb; j0 < ce; j0 += wave_size)
+  {
+    // one batch of up to 64 columns: ids, packed 1D indices and dimension-2
+    // factor offsets, one column per lane
+    const int64_t jl = j0 + lane;
+    int colv = 0, off2v = 0;
+    unsigned int cplo = 0;
+    if (jl < ce)
+    {
+      colv = a.indx[jl];
+      const unsigned long long w = a.epack[colv];
+      cplo = static_cast<unsigned int>(w & 0x3FFFFFFFull);
+      off2v = a.off1d[r2M + packed_1d(w, 2)];
+    }
+    const int nb = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
+    // software pipeline: the x entry and stage-1 factors of column jj + 1
+    // are loaded before column jj is contracted
+    double xn = 0, fn[TT];
+    auto fetch = [&](int jj, double &xv, double (&fv)[TT]) {
+      const int col = __builtin_amdgcn_readlane(colv, jj);
+      const int off2 = __builtin_amdgcn_readlane(off2v, jj);
+      const int64_t base = int64_t(off2) * blk + 18;
+      xv = xvalid ? a.x[int64_t(col) * N + xoff] : 0.0;
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+        if (t < T)
+          fv[t] = fvalid ? a.vA[base + t * 27 + foff] : 0.0;
+    };
+    fetch(0, xn, fn);
+    for (int jj = 0; jj < nb; jj++)
+    {
+      const double xv = xn;
+      double fv[TT];
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+        fv[t] = fn[t];
+      if (jj + 1 < nb)
+        fetch(jj + 1, xn, fn);
+      const unsigned int cp = static_cast<unsigned int>(__builtin_amdgcn_readlane(cplo, jj));
+      const unsigned int k01 = cp & 0xFFFFFu, k0 = cp & 0x3FFu;
+      if (k01 != prev01)
+      {
+        if (prev01 != 0xFFFFFFFFu)
+          close_01();
+        if (k0 != prev0)
+        {
+          if (prev0 != 0xFFFFFFFFu)
+            close_0();
+          load_f0(static_cast<int>(k0));
+          prev0 = k0;
+        }
+        load_f1(static_cast<int>((cp >> 10) & 1023u));
+        prev01 = k01;
+      }
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+        if (t < T)
+          s1[t] = mfma4(xv, fv[t], s1[t]);
+    }
+  }
+  if (prev01 != 0xFFFFFFFFu)
+  {
+    close_01();
+    close_0();
+  }
+
+  // the wavefronts' sums in wavefront order; lane (q = j1, g = j0, m = j2)
+  red[wave * wave_size + lane] = acc;
+  __syncthreads();
+  const int64_t ybase = int64_t(row) * N;
+  for (int e = threadIdx.x; e < N; e += W * wave_size)
+  {
+    const int j0 = e / 9, j1 = (e / 3) % 3, j2 = e % 3;
+    const int src = 16 * j1 + 4 * j0 + j2;
+    double tot = red[src];
+    for (int w = 1; w < W; w++)
+      tot += red[w * wave_size + src];
+    store_row(a, ybase + e, tot);
+  }
+}
+
+template<int TT>
+void launch_mfma_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
+{
+  switch (waves)
+  {
+  case 1: kron_mfma_d3n3<1, TT><<<num_rows, 1 * wave_size, 0, stream>>>(a); break;
+  case 2: kron_mfma_d3n3<2, TT><<<num_rows, 2 * wave_size, 0, stream>>>(a); break;
+  case 8: kron_mfma_d3n3<8, TT><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
+  default: kron_mfma_d3n3<4, TT><<<num_rows, 4 * wave_size, 0, stream>>>(a); break;
+  }
+}
+
+// ---------------------------------------------------------- d = 6, n = 2 ---
+//
+// With n = 2 two dimensions together form a dense 4 x 4 factor, the
+// Kronecker product of their 2 x 2 blocks, so one 4 x 4 x 4 product contracts
+// a pair of dimensions with no padding.  One wavefront holds one 64-entry
+// tensor, entry e = 16 g + 4 m + q in lane 16 q + 4 g + m, i.e. dimensions
+// (0, 1) on the block g, (2, 3) on m and (4, 5) on q:
+//   stage 1 (dims 4, 5):  A = x (i = (i2, i3) = m, k = (i4, i5) = q),
+//                         B[k][j] = A4(j4, i4) A5(j5, i5), j = (j4, j5) = m
+//                         -> D1[i = (i2, i3) = q][j = (j4, j5) = m]
+//   stage 2 (dims 2, 3):  A[i][k] = A2(j2, i2) A3(j3, i3) (i = m, k = q),
+//                         B = D1 -> D2[(j2, j3) = q][(j4, j5) = m]
+//   stage 3 (dims 0, 1):  lane (q, g' = (j0, j1), m) sums
+//                         A0(j0, i0) A1(j1, i1) D2 over the blocks g = (i0, i1).
+// The row's columns are in lexicographic order of (c0, ..., c5): stage 1 runs
+// per column and its results are summed over the columns sharing
+// (c0, c1, c2, c3); stage 2 runs once per such group and is summed over the
+// groups sharing (c0, c1); stage 3 runs once per (c0, c1) group.  The
+// dimension-4 block is reloaded only when (c0, ..., c4) changes.
+
+//! Element of a 2 x 2 column-major block pair product for lane (q, m):
+//! A(jhi, ihi) B(jlo, ilo) with (ihi, ilo) = bits of q, (jhi, jlo) = bits of m.
+__device__ __forceinline__ int pair_hi(int q, int m) { return (q >> 1) * 2 + (m >> 1); }
+__device__ __forceinline__ int pair_lo(int q, int m) { return (q & 1) * 2 + (m & 1); }
+
+template<int W, int TT>
+__global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double> a)
+{
+  constexpr int N = 64;
+  __shared__ double red[W * wave_size];
+  const int lane = threadIdx.x & (wave_size - 1);
+  const int wave = threadIdx.x / wave_size;
+  const int q = lane >> 4, g = (lane >> 2) & 3, m = lane & 3;
+  const int T = a.T;
+
+  const int row = a.row_order[blockIdx.x];
+  const int64_t jbeg = a.pntr[row], jend = a.pntr[row + 1];
+  const int64_t len = jend - jbeg;
+  const int64_t cb = jbeg + (len * wave) / W, ce = jbeg + (len * (wave + 1)) / W;
+
+  int rM[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+    rM[k] = a.elem[int64_t(row) * 6 + k] * a.M;
+  const int64_t blk = int64_t(T) * 24; // one connection: T terms x 6 dims x 4
+  const int xoff = 16 * g + 4 * m + q;
+  const int ehi = pair_hi(q, m), elo = pair_lo(q, m);
+
+  double s1[TT], s2[TT], a4[TT], k23[TT], k01[TT][4];
+#pragma unroll
+  for (int t = 0; t < TT; t++)
+  {
+    s1[t] = s2[t] = a4[t] = k23[t] = 0;
+    k01[t][0] = k01[t][1] = k01[t][2] = k01[t][3] = 0;
+  }
+  double acc = 0;
+
+  auto block_of = [&](int k, int c) -> int64_t {
+    return int64_t(a.off1d[rM[k] + c]) * blk + k * 4;
+  };
+  auto load_a4 = [&](int c4) {
+    const int64_t b4 = block_of(4, c4);
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+        a4[t] = a.vA[b4 + t * 24 + ehi];
+  };
+  auto load_k23 = [&](int c2, int c3) {
+    const int64_t b2 = block_of(2, c2), b3 = block_of(3, c3);
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+        k23[t] = a.vA[b2 + t * 24 + ehi] * a.vA[b3 + t * 24 + elo];
+  };
+  // A0(j0, i0) A1(j1, i1) for output block g' = (j0, j1) and each source
+  // block (i0, i1)
+  auto load_k01 = [&](int c0, int c1) {
+    const int64_t b0 = block_of(0, c0), b1 = block_of(1, c1);
+    const int j0 = g >> 1, j1 = g & 1;
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+      {
+        const double p00 = a.vA[b0 + t * 24 + 0 * 2 + j0], p01 = a.vA[b0 + t * 24 + 1 * 2 + j0];
+        const double p10 = a.vA[b1 + t * 24 + 0 * 2 + j1], p11 = a.vA[b1 + t * 24 + 1 * 2 + j1];
+        k01[t][0] = p00 * p10; // (i0, i1) = (0, 0)
+        k01[t][1] = p00 * p11; // (0, 1)
+        k01[t][2] = p01 * p10; // (1, 0)
+        k01[t][3] = p01 * p11; // (1, 1)
+      }
+  };
+  auto close_4 = [&]() {
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+      {
+        s2[t] = mfma4(k23[t], s1[t], s2[t]);
+        s1[t] = 0;
+      }
+  };
+  auto close_2 = [&]() {
+    const int src = 16 * q + m;
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+      if (t < T)
+      {
+        double v = acc;
+#pragma unroll
+        for (int gs = 0; gs < 4; gs++)
+          v = fma(k01[t][gs], __shfl(s2[t], src + 4 * gs, wave_size), v);
+        acc = v;
+        s2[t] = 0;
+      }
+  };
+
+  constexpr unsigned long long mask2 = (1ull << 20) - 1, mask4 = (1ull << 40) - 1,
+                               mask5 = (1ull << 50) - 1, none = ~0ull;
+  unsigned long long prev2 = none, prev4 = none, prev5 = none;
+  for (int64_t j0 = cb; j0 < ce; j0 += wave_size)
+  {
+    const int64_t jl = j0 + lane;
+    int colv = 0, off5v = 0;
+    unsigned long long wv = 0;
+    if (jl < ce)
+    {
+      colv = a.indx[jl];
+      wv = a.epack[colv];
+      off5v = a.off1d[rM[5] + packed_1d(wv, 5)];
+    }
+    const unsigned int wlo = static_cast<unsigned int>(wv), whi = static_cast<unsigned int>(wv >> 32);
+    const int nb = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
+    double xn = 0, fn[TT];
+    auto fetch = [&](int jj, double &xv, double (&fv)[TT]) {
+      const int col = __builtin_amdgcn_readlane(colv, jj);
+      const int64_t b5 = int64_t(__builtin_amdgcn_readlane(off5v, jj)) * blk + 20;
+      xv = a.x[int64_t(col) * N + xoff];
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+        if (t < T)
+          fv[t] = a.vA[b5 + t * 24 + elo];
+    };
+    fetch(0, xn, fn);
+    for (int jj = 0; jj < nb; jj++)
+    {
+      const double xv = xn;
+      double f5[TT];
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+        f5[t] = fn[t];
+      if (jj + 1 < nb)
+        fetch(jj + 1, xn, fn);
+      const unsigned long long w =
+          (static_cast<unsigned long long>(
+               static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
+           << 32) |
+          static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
+      const unsigned long long p5 = w & mask5;
+      if (p5 != prev5)
+      {
+        const unsigned long long p4 = w & mask4;
+        if (p4 != prev4)
+        {
+          if (prev4 != none)
+            close_4();
+          const unsigned long long p2 = w & mask2;
+          if (p2 != prev2)
+          {
+            if (prev2 != none)
+              close_2();
+            load_k01(packed_1d(w, 0), packed_1d(w, 1));
+            prev2 = p2;
+          }
+          load_k23(packed_1d(w, 2), packed_1d(w, 3));
+          prev4 = p4;
+        }
+        load_a4(packed_1d(w, 4));
+        prev5 = p5;
+      }
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+        if (t < T)
+          s1[t] = mfma4(xv, a4[t] * f5[t], s1[t]);
+    }
+  }
+  if (prev4 != none)
+  {
+    close_4();
+    close_2();
+  }
+
+  // lane (q = (j2, j3), g = (j0, j1), m = (j4, j5)) holds entry 16 g + 4 q + m
+  red[wave * wave_size + lane] = acc;
+  __syncthreads();
+  const int64_t ybase = int64_t(row) * N;
+  for (int e = threadIdx.x; e < N; e += W * wave_size)
+  {
+    const int src = 16 * ((e >> 2) & 3) + 4 * (e >> 4) + (e & 3);
+    double tot = red[src];
+    for (int w = 1; w < W; w++)
+      tot += red[w * wave_size + src];
+    store_row(a, ybase + e, tot);
+  }
+}
+
+template<int TT>
+void launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
+{
+  switch (waves)
+  {
+  case 1: kron_mfma_d6n2<1, TT><<<num_rows, 1 * wave_size, 0, stream>>>(a); break;
+  case 2: kron_mfma_d6n2<2, TT><<<num_rows, 2 * wave_size, 0, stream>>>(a); break;
+  case 8: kron_mfma_d6n2<8, TT><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
+  default: kron_mfma_d6n2<4, TT><<<num_rows, 4 * wave_size, 0, stream>>>(a); break;
+  }
+}
+
+inline bool launch_mfma_d6n2(kron_args<double> const &a, int num_rows, int waves,
+                             hipStream_t stream)
+{
+  if (a.T > mfma_max_terms || a.epack == nullptr)
+    return false;
+  if (a.T <= 2)
+    launch_mfma6_terms<2>(a, num_rows, waves, stream);
+  else if (a.T <= 4)
+    launch_mfma6_terms<4>(a, num_rows, waves, stream);
+  else if (a.T <= 6)
+    launch_mfma6_terms<6>(a, num_rows, waves, stream);
+  else
+    launch_mfma6_terms<mfma_max_terms>(a, num_rows, waves, stream);
+  return true;
+}
+
+// ---------------------------------------------------------- d = 4, n = 2 ---
+//
+// A 16-entry tensor fills one block of the 4 x 4 x 4 product, so the four
+// blocks carry four terms of the same column at once (term 4 p + g in block
+// g for term group p).  Entry e = 4 m + q of the column sits in lane
+// 16 q + 4 g + m of every block:
+//   stage 1 (dims 2, 3):  A = x (i = (i0, i1) = m, k = (i2, i3) = q),
+//                         B[k][j] = A2(j2, i2) A3(j3, i3) of the block's term
+//                         -> D1[(i0, i1) = q][(j2, j3) = m]
+//   stage 2 (dims 0, 1):  A[i][k] = A0(j0, i0) A1(j1, i1) (i = m, k = q),
+//                         B = D1 -> D2[(j0, j1) = q][(j2, j3) = m]
+// Stage 1 runs per column and is summed over the columns sharing (c0, c1);
+// stage 2 runs once per such group.  The terms (blocks) are summed at the end.
+template<int W, int TG>
+__global__ void __launch_bounds__(W *wave_size) kron_mfma_d4n2(kron_args<double> a)
+{
+  constexpr int N = 16;
+  __shared__ double red[W * wave_size];
+  const int lane = threadIdx.x & (wave_size - 1);
+  const int wave = threadIdx.x / wave_size;
+  const int q = lane >> 4, g = (lane >> 2) & 3, m = lane & 3;
+  const int T = a.T;
+
+  const int row = a.row_order[blockIdx.x];
+  const int64_t jbeg = a.pntr[row], jend = a.pntr[row + 1];
+  const int64_t len = jend - jbeg;
+  const int64_t cb = jbeg + (len * wave) / W, ce = jbeg + (len * (wave + 1)) / W;
+
+  int rM[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    rM[k] = a.elem[int64_t(row) * 4 + k] * a.M;
+  const int64_t blk = int64_t(T) * 16; // one connection: T terms x 4 dims x 4
+  const int xoff = 4 * m + q;
+  const int ehi = pair_hi(q, m), elo = pair_lo(q, m);
+  // term of this lane's block in each term group (-1: no such term)
+  int term[TG];
+#pragma unroll
+  for (int p = 0; p < TG; p++)
+    term[p] = (4 * p + g < T) ? 4 * p + g : -1;
+
+  double s1[TG], s2[TG], a2[TG], k01[TG];
+#pragma unroll
+  for (int p = 0; p < TG; p++)
+    s1[p] = s2[p] = a2[p] = k01[p] = 0;
+
+  auto block_of = [&](int k, int c) -> int64_t {
+    return int64_t(a.off1d[rM[k] + c]) * blk + k * 4;
+  };
+  auto load_a2 = [&](int c2) {
+    const int64_t b2 = block_of(2, c2);
+#pragma unroll
+    for (int p = 0; p < TG; p++)
+      a2[p] = (term[p] >= 0) ? a.vA[b2 + term[p] * 16 + ehi] : 0.0;
+  };
+  auto load_k01 = [&](int c0, int c1) {
+    const int64_t b0 = block_of(0, c0), b1 = block_of(1, c1);
+#pragma unroll
+    for (int p = 0; p < TG; p++)
+      k01[p] = (term[p] >= 0)
+                   ? a.vA[b0 + term[p] * 16 + ehi] * a.vA[b1 + term[p] * 16 + elo]
+                   : 0.0;
+  };
+  auto close_2 = [&]() {
+#pragma unroll
+    for (int p = 0; p < TG; p++)
+    {
+      s2[p] = mfma4(k01[p], s1[p], s2[p]);
+      s1[p] = 0;
+    }
+  };
+
+  constexpr unsigned long long mask2 = (1ull << 20) - 1, mask3 = (1ull << 30) - 1, none = ~0ull;
+  unsigned long long prev2 = none, prev3 = none;
+  for (int64_t j0 = cb; j0 < ce; j0 += wave_size)
+  {
+    const int64_t jl = j0 + lane;
+    int colv = 0, off3v = 0;
+    unsigned long long wv = 0;
+    if (jl < ce)
+    {
+      colv = a.indx[jl];
+      wv = a.epack[colv];
+      off3v = a.off1d[rM[3] + packed_1d(wv, 3)];
+    }
+    const unsigned int wlo = static_cast<unsigned int>(wv), whi = static_cast<unsigned int>(wv >> 32);
+    const int nb = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
+    double xn = 0, fn[TG];
+    auto fetch = [&](int jj, double &xv, double (&fv)[TG]) {
+      const int col = __builtin_amdgcn_readlane(colv, jj);
+      const int64_t b3 = int64_t(__builtin_amdgcn_readlane(off3v, jj)) * blk + 12;
+      xv = a.x[int64_t(col) * N + xoff];
+#pragma unroll
+      for (int p = 0; p < TG; p++)
+        fv[p] = (term[p] >= 0) ? a.vA[b3 + term[p] * 16 + elo] : 0.0;
+    };
+    fetch(0, xn, fn);
+    for (int jj = 0; jj < nb; jj++)
+    {
+      const double xv = xn;
+      double f3[TG];
+#pragma unroll
+      for (int p = 0; p < TG; p++)
+        f3[p] = fn[p];
+      if (jj + 1 < nb)
+        fetch(jj + 1, xn, fn);
+      const unsigned long long w =
+          (static_cast<unsigned long long>(
+               static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
+           << 32) |
+          static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
+      const unsigned long long p3 = w & mask3;
+      if (p3 != prev3)
+      {
+        const unsigned long long p2 = w & mask2;
+        if (p2 != prev2)
+        {
+          if (prev2 != none)
+            close_2();
+          load_k01(packed_1d(w, 0), packed_1d(w, 1));
+          prev2 = p2;
+        }
+        load_a2(packed_1d(w, 2));
+        prev3 = p3;
+      }
+#pragma unroll
+      for (int p = 0; p < TG; p++)
+        s1[p] = mfma4(xv, a2[p] * f3[p], s1[p]);
+    }
+  }
+  if (prev2 != none)
+    close_2();
+
+  // lane (q = (j0, j1), g = term, m = (j2, j3)) holds entry 4 q + m of its
+  // blocks' terms; the terms are summed in block order
+  double acc = 0;
+#pragma unroll
+  for (int p = 0; p < TG; p++)
+    acc += s2[p];
+  red[wave * wave_size + lane] = acc;
+  __syncthreads();
+  const int64_t ybase = int64_t(row) * N;
+  for (int e = threadIdx.x; e < N; e += W * wave_size)
+  {
+    const int src = 16 * (e >> 2) + (e & 3);
+    double tot = 0;
+    for (int w = 0; w < W; w++)
+      for (int gg = 0; gg < 4; gg++)
+        tot += red[w * wave_size + src + 4 * gg];
+    store_row(a, ybase + e, tot);
+  }
+}
+
+template<int TG>
+void launch_mfma4_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
+{
+  switch (waves)
+  {
+  case 1: kron_mfma_d4n2<1, TG><<<num_rows, 1 * wave_size, 0, stream>>>(a); break;
+  case 2: kron_mfma_d4n2<2, TG><<<num_rows, 2 * wave_size, 0, stream>>>(a); break;
+  case 8: kron_mfma_d4n2<8, TG><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
+  default: kron_mfma_d4n2<4, TG><<<num_rows, 4 * wave_size, 0, stream>>>(a); break;
+  }
+}
+
+inline bool launch_mfma_d4n2(kron_args<double> const &a, int num_rows, int waves,
+                             hipStream_t stream)
+{
+  if (a.T > mfma_max_terms || a.epack == nullptr)
+    return false;
+  if (a.T <= 4)
+    launch_mfma4_terms<1>(a, num_rows, waves, stream);
+  else
+    launch_mfma4_terms<2>(a, num_rows, waves, stream);
+  return true;
+}
+
+//! Launches the sum-factorised matrix-core kernel; false when it does not
+//! apply (other (d, n), more than mfma_max_terms terms, not sparse-grid).
+inline bool launch_mfma_d3n3(kron_args<double> const &a, int num_rows, int waves,
+                             hipStream_t stream)
+{
+  if (a.T > mfma_max_terms || a.epack == nullptr)
+    return false;
+  if (a.T <= 4)
+    launch_mfma_terms<4>(a, num_rows, waves, stream);
+  else
+    launch_mfma_terms<mfma_max_terms>(a, num_rows, waves, stream);
+  return true;
+}
+
+} // namespace akm

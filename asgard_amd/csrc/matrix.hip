@@ -141,6 +141,7 @@ static kron_args<P> matrix_args(akm_matrix_s const &m)
   a.max_table_elems = m.max_table_elems;
   a.lds_cap = (m.vAk.data() != nullptr) ? lds_cap_bytes(m) : 0;
   a.row_order = m.row_order.data();
+  a.family = m.family;
   return a;
 }
 
@@ -371,10 +372,13 @@ akm_status akm_create_sparse_grid(int d, int n, int64_t num_cells, const int *ta
 
   std::unique_ptr<akm_matrix_s> m(new_matrix(d, n, int(num_cells), int(num_cells), T, prec));
   m->mode = mode_grid;
-  AKM_TRY(device_sparse_pattern(d, num_cells, table, m->pntr, m->indx));
-  m->nnz = m->pntr[num_cells];
-  std::vector<int> elem;
+  std::vector<int> elem, lex;
   cell_1d_indices(d, num_cells, table, elem);
+  // each row's columns in lexicographic order of their 1D indices: the
+  // sum-factorised kernels close a group of columns when its prefix changes
+  lexicographic_order(d, num_cells, elem.data(), lex);
+  AKM_TRY(device_sparse_pattern(d, num_cells, table, m->pntr, m->indx, lex.data()));
+  m->nnz = m->pntr[num_cells];
   AKM_TRY(m->elem.upload(elem.data(), int64_t(elem.size())));
   AKM_TRY(m->off1d.upload(offtab.data(), int64_t(offtab.size())));
   m->M = 1 << max_level;
@@ -589,13 +593,13 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
   }
   struct choice
   {
-    int waves, lds;
+    int waves, lds, family;
   };
   static std::mutex mtx;
   static std::map<std::vector<int64_t>, choice> cache;
   const std::vector<int64_t> key = {int64_t(mat->prec), mat->d,      mat->n,  mat->T,
                                     mat->max_level,     mat->rows,   mat->nnz, int64_t(mat->mode)};
-  choice best{0, -1};
+  choice best{0, -1, 0};
   bool cached = false;
   {
     std::lock_guard<std::mutex> lock(mtx);
@@ -612,6 +616,7 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
     double probe = 0;
     mat->waves = 4;
     mat->lds_cap = -1;
+    mat->family = 0;
     AKM_TRY(akm_time_apply(mat, 1.0, x, 0.0, y, 1, &probe));
     const int iters = std::max(1, std::min(50, int(2.0 / std::max(probe, 1e-3))));
     double best_ms = 1e300;
@@ -626,18 +631,53 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
         if (ms < best_ms)
         {
           best_ms = ms;
-          best = {w, mat->lds_cap};
+          best = {w, mat->lds_cap, 0};
         }
       }
+    if (mat->prec == AKM_F64 &&
+        mfma_family_applies(mat->d, mat->n, mat->mode, mat->T, mat->epack.data() != nullptr))
+    {
+      const int saved_lds = mat->lds_cap;
+      mat->family = 1;
+      for (int w : {1, 2, 4, 8})
+      {
+        mat->waves = w;
+        double ms = 0;
+        AKM_TRY(akm_time_apply(mat, 1.0, x, 0.0, y, iters, &ms));
+        if (ms < best_ms)
+        {
+          best_ms = ms;
+          best = {w, saved_lds, 1};
+        }
+      }
+      mat->family = 0;
+    }
     std::lock_guard<std::mutex> lock(mtx);
     cache[key] = best;
   }
   mat->waves = best.waves;
   mat->lds_cap = best.lds;
+  mat->family = best.family;
   if (waves_per_row)
     *waves_per_row = best.waves;
   if (lds_tables)
-    *lds_tables = (best.lds != 0 && mat->vAk.data() != nullptr) ? 1 : 0;
+    *lds_tables = (best.family == 1) ? 2 : (best.lds != 0 && mat->vAk.data() != nullptr) ? 1 : 0;
+  return AKM_SUCCESS;
+}
+
+akm_status akm_set_kernel_family(akm_matrix mat, int family)
+{
+  if (mat == nullptr)
+    return set_error(AKM_ERR_INVALID, "kernel family: null matrix");
+  if (family != 0 && family != 1)
+    return set_error(AKM_ERR_INVALID, "kernel family must be 0 or 1 (got %d)", family);
+  if (family == 1 && !(mat->prec == AKM_F64 &&
+                       mfma_family_applies(mat->d, mat->n, mat->mode, mat->T,
+                                           mat->epack.data() != nullptr)))
+    return set_error(AKM_ERR_UNSUPPORTED,
+                     "kernel family 1 needs an FP64 sparse-grid matrix with d = 3, n = 3, "
+                     "1 to 8 terms and max_level <= 10");
+  mat->family = family;
   return AKM_SUCCESS;
 }
 

@@ -45,6 +45,7 @@ struct akm_matrix_s
   // work decomposition: one workgroup per row, rows longest first
   int waves = 0;        // wavefronts per row workgroup (0 = automatic)
   int force_generic = 0;
+  int family = 0;       // 0 register / LDS-table kernels, 1 matrix-core kernel
   akm::dev_buffer<int64_t> pntr_dev;
   akm::dev_buffer<int> row_order;
 

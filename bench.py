@@ -138,7 +138,7 @@ def profiled_traffic(profile, waves, lds_tables):
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (each counter scaled by what it
     reads for a plain device copy of known size); None when the shape was not
     profiled."""
-    key = "traffic_bytes W=%d lds=%d:" % (waves, 1 if lds_tables else 0)
+    key = "traffic_bytes W=%d lds=%d:" % (waves, int(lds_tables))
     try:
         with open(os.path.join(ROOT, "profiles", profile)) as f:
             for line in f:

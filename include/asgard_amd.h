@@ -199,13 +199,25 @@ akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
 
 /* Autotuner: times every workgroup shape (1, 2, 4, 8 wavefronts per row)
  * with and without the LDS factor tables on the matrix itself (device x/y,
- * y is overwritten) and keeps the fastest.  Choices are cached per
- * (precision, d, n, terms, max level, rows, pairs) for the process, so a
- * second matrix of the same shape reuses them without timing.  Writes the
- * chosen waves per row and LDS switch (1 = tables) when the pointers are not
+ * y is overwritten), and the sum-factorised matrix-core kernel where it
+ * applies (akm_set_kernel_family), and keeps the fastest.  Choices are
+ * cached per (precision, d, n, terms, max level, rows, pairs) for the
+ * process, so a second matrix of the same shape reuses them without timing.
+ * Writes the chosen waves per row and kernel (0 = factors read from global
+ * memory, 1 = LDS factor tables, 2 = matrix cores) when the pointers are not
  * NULL.  No counterpart in the reference (fixed launch shapes). */
 akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_row,
                         int *lds_tables);
+
+/* Kernel family of the apply: 0 = the register / LDS-table kernels, 1 = the
+ * sum-factorised matrix-core kernel for FP64 sparse-grid matrices with
+ * d = 3, n = 3, 1 to 8 terms and max_level <= 10 (the row's columns taken in
+ * lexicographic order of their 1D indices; dimensions 2 and 1 contracted by
+ * v_mfma_f64_4x4x4_4b_f64 with the group sums in its accumulator, dimension 0
+ * once per group).  AKM_ERR_UNSUPPORTED for any other matrix.  Takes effect
+ * at the next apply; akm_set_tuning's waves_per_row sets its wavefronts per
+ * row.  No counterpart in the reference. */
+akm_status akm_set_kernel_family(akm_matrix mat, int family);
 
 /* Times `iterations` back-to-back applies (device x/y) with HIP events on the
  * matrix's stream; returns the average milliseconds per apply. */

@@ -324,3 +324,140 @@ def test_transposed_exchange_parity(d, n, L, T, waves):
         y = dev(c["y"].astype(dtype))
         mi.apply(1.5, dev(c["x"].astype(dtype)), -0.5, y)
         assert rel(host(y), ref) < tol, (dtype, "ia")
+
+
+# ------------------------------ sum-factorised matrix-core kernel (d=3, n=3) ---
+
+@pytest.mark.parametrize("L,T,seed", [(2, 1, 1), (4, 3, 2), (6, 3, 0), (5, 4, 3), (4, 5, 4),
+                                      (3, 8, 5)])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_mfma_kernel_parity(L, T, seed, waves):
+    """akm_set_kernel_family(1): dimensions 2 and 1 on v_mfma_f64_4x4x4_4b_f64
+    with the group sums in the accumulator; against the oracle at every
+    workgroup shape and both term bounds (T <= 4, T <= 8)."""
+    c = sparse_grid_case(3, 3, L, T, seed=seed, dist="normal")
+    m = asgard_amd.KronmultMatrix.sparse_grid(3, 3, c["table"], L, T, c["vA"])
+    m.set_kernel_family(1)
+    m.set_tuning(waves_per_row=waves)
+    y = dev(c["y"])
+    m.apply(0.75, dev(c["x"]), -0.5, y)
+    assert rel(host(y), oracle_apply(c, 0.75, -0.5)) < TOL64
+    # the same apply again repeats bit for bit
+    y2 = dev(c["y"])
+    m.apply(0.75, dev(c["x"]), -0.5, y2)
+    assert torch.equal(y, y2)
+
+
+def test_mfma_kernel_dg_operators_and_euler():
+    from asgard_amd import dg
+
+    L, T = 5, 3
+    ops = dg.operators_advection(3, 3, L)
+    c = sparse_grid_case(3, 3, L, T, seed=6, ops=ops)
+    m = asgard_amd.KronmultMatrix.sparse_grid(3, 3, c["table"], L, T, c["vA"])
+    m.set_kernel_family(1)
+    x = dev(c["x"])
+    y = m.apply_euler(0.01, 1.0, x, 0.0, dev(np.zeros_like(c["x"])))
+    ax = oracle_apply(c, 1.0, 0.0)
+    assert rel(host(y), c["x"] - 0.01 * ax) < TOL64
+
+
+def test_mfma_kernel_refined_grid():
+    from asgard_amd import grid
+
+    table, _ = oracle.element_table([4] * 3)
+    table = grid.refine(table, 0.3, seed=2)
+    L = int(table[:, :3].max())
+    rng = np.random.default_rng(9)
+    s = 3 << L
+    ops = [np.asfortranarray(rng.uniform(-1, 1, (s, s))) for _ in range(2 * 3)]
+    vA = oracle.build_vA(3, 3, 2, L, ops)
+    pntr, indx, iA = oracle.sparse_pattern(table, n=3, T=2, max_level=L)
+    x = rng.uniform(-1, 1, table.shape[0] * 27)
+    ref = np.zeros_like(x)
+    oracle.kronmult_sparse(3, 3, pntr, indx, 2, iA, vA, 1.0, x, 0.0, ref)
+    m = asgard_amd.KronmultMatrix.sparse_grid(3, 3, table, L, 2, vA)
+    m.set_kernel_family(1)
+    y = dev(np.zeros_like(x))
+    m.apply(1.0, dev(x), 0.0, y)
+    assert rel(host(y), ref) < TOL64
+
+
+@pytest.mark.parametrize("L,T,seed", [(2, 1, 1), (3, 2, 2), (3, 5, 3), (4, 6, 4), (3, 8, 5)])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_mfma_d6n2_kernel_parity(L, T, seed, waves):
+    """d=6, n=2: dimensions paired into dense 4 x 4 Kronecker factors on
+    v_mfma_f64_4x4x4_4b_f64; against the oracle for every term bound."""
+    c = sparse_grid_case(6, 2, L, T, seed=seed, dist="normal")
+    m = asgard_amd.KronmultMatrix.sparse_grid(6, 2, c["table"], L, T, c["vA"])
+    m.set_kernel_family(1)
+    m.set_tuning(waves_per_row=waves)
+    y = dev(c["y"])
+    m.apply(-1.25, dev(c["x"]), 0.5, y)
+    assert rel(host(y), oracle_apply(c, -1.25, 0.5)) < TOL64
+    y2 = dev(c["y"])
+    m.apply(-1.25, dev(c["x"]), 0.5, y2)
+    assert torch.equal(y, y2)
+
+
+@pytest.mark.parametrize("L,T,seed", [(2, 1, 1), (4, 3, 2), (5, 4, 3), (4, 6, 4), (3, 8, 5)])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_mfma_d4n2_kernel_parity(L, T, seed, waves):
+    """d=4, n=2: the terms in the four blocks of the 4 x 4 x 4 product, both
+    paired-dimension stages on the matrix cores; against the oracle."""
+    c = sparse_grid_case(4, 2, L, T, seed=seed, dist="normal")
+    m = asgard_amd.KronmultMatrix.sparse_grid(4, 2, c["table"], L, T, c["vA"])
+    m.set_kernel_family(1)
+    m.set_tuning(waves_per_row=waves)
+    y = dev(c["y"])
+    m.apply(0.5, dev(c["x"]), 1.0, y)
+    assert rel(host(y), oracle_apply(c, 0.5, 1.0)) < TOL64
+
+
+def test_mfma_d4n2_advdiff_euler():
+    from asgard_amd import dg
+
+    L, T = 5, 4
+    ops = dg.operators_advection_diffusion(4, 2, L, n_adv=2, nu=0.1)
+    c = sparse_grid_case(4, 2, L, T, seed=7, ops=ops)
+    m = asgard_amd.KronmultMatrix.sparse_grid(4, 2, c["table"], L, T, c["vA"])
+    m.set_kernel_family(1)
+    y = m.apply_euler(1e-3, 1.0, dev(c["x"]), 0.0, dev(np.zeros_like(c["x"])))
+    assert rel(host(y), c["x"] - 1e-3 * oracle_apply(c, 1.0, 0.0)) < TOL64
+
+
+def test_mfma_d6n2_vlasov_sampled_rows():
+    """The d=6 L=7 Vlasov operator through the matrix-core kernel, checked on
+    sampled rows of every length against the oracle."""
+    from asgard_amd import dg, grid
+
+    d, n, L = 6, 2, 7
+    table, _ = grid.element_table([L] * d)
+    ops = dg.operators_vlasov(n, L)
+    T = len(ops) // d
+    vA = grid.build_vA(d, n, T, L, ops)
+    m = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
+    m.set_kernel_family(1)
+    x = np.random.default_rng(0).uniform(-1, 1, table.shape[0] * n ** d)
+    y = dev(np.zeros_like(x))
+    m.apply(1.0, dev(x), 0.0, y)
+    rows = np.linspace(0, table.shape[0] - 1, 12).astype(np.int32)
+    pntr, indx, iA = oracle.sparse_pattern_rows(table, rows, n=n, T=T, max_level=L)
+    ref = np.zeros(len(rows) * n ** d)
+    oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, ref)
+    got = host(y).reshape(-1, n ** d)[rows].reshape(-1)
+    assert rel(got, ref) < TOL64
+
+
+def test_mfma_kernel_family_rejected_elsewhere():
+    c = sparse_grid_case(5, 2, 3, 1, seed=7)
+    m = asgard_amd.KronmultMatrix.sparse_grid(5, 2, c["table"], 3, 1, c["vA"])
+    with pytest.raises(asgard_amd.AkmError):
+        m.set_kernel_family(1)
+    c3 = sparse_grid_case(3, 3, 3, 2, seed=8)
+    m32 = asgard_amd.KronmultMatrix.sparse_grid(3, 3, c3["table"], 3, 2,
+                                                c3["vA"].astype(np.float32), dtype=np.float32)
+    with pytest.raises(asgard_amd.AkmError):
+        m32.set_kernel_family(1)
+    with pytest.raises(ValueError):
+        m.set_kernel_family(2)
