@@ -49,7 +49,7 @@ EXPORTS = (
     "akm_update_operators", "akm_imex_step", "akm_dg_term",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
     "akm_autotune", "akm_set_kernel_family", "akm_time_apply", "akm_destroy", "akm_gmres_euler",
-    "akm_gmres_dense",
+    "akm_gmres_euler_noalloc", "akm_gmres_dense",
     "akm_gmres_euler_mixed", "akm_mlp_create", "akm_mlp_info", "akm_mlp_set_stream",
     "akm_mlp_get_params", "akm_mlp_set_params", "akm_mlp_forward", "akm_mlp_gradients",
     "akm_mlp_adam_step", "akm_mlp_train", "akm_precond_apply", "akm_precond_destroy",
@@ -57,6 +57,9 @@ EXPORTS = (
 )
 
 GmresInfo = namedtuple("GmresInfo", ["error", "iterations"])
+
+#: AKM_NOALLOC_SCRATCH_BYTES of include/asgard_amd.h
+NOALLOC_SCRATCH_BYTES = 32 << 20
 
 class DgPartial(ctypes.Structure):
     """akm_dg_partial: one partial term of a 1D DG operator."""
@@ -147,6 +150,8 @@ def load_library(path: str = LIB_PATH):
     L.akm_destroy.argtypes = [_vp]
     L.akm_gmres_euler.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
                                   ctypes.POINTER(_i)]
+    L.akm_gmres_euler_noalloc.argtypes = [_vp, _d, _vp, _vp, _i, _i, _d, ctypes.POINTER(_d),
+                                          ctypes.POINTER(_i)]
     L.akm_gmres_dense.argtypes = [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _i, ctypes.POINTER(_d),
                                   ctypes.POINTER(_i)]
     L.akm_gmres_euler_mixed.argtypes = [_vp, _vp, _vp, _d, _vp, _vp, _i, _i, _d, _d,
@@ -580,6 +585,23 @@ def simple_gmres_euler(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, t
     _check(load_library().akm_gmres_euler(mat._h, float(dt), px, pb, int(restart),
                                           int(max_iter), float(tol), dx, ctypes.byref(err),
                                           ctypes.byref(its)), allow_not_converged=True)
+    return GmresInfo(err.value, its.value)
+
+
+def gmres_euler_noalloc(dt, mat: KronmultMatrix, x, b, restart=-1, max_iter=-1, tol=-1.0):
+    """simple_gmres_euler on device vectors with no device allocation: the
+    solver's memory is a fixed per-GPU array of AKM_NOALLOC_SCRATCH_BYTES
+    (NOALLOC_SCRATCH_BYTES here).  Same results as simple_gmres_euler;
+    raises AkmInvalid when the system and restart do not fit."""
+    px, dx = _vec_ptr(x, mat.dtype)
+    pb, db = _vec_ptr(b, mat.dtype)
+    if not (dx and db):
+        raise AkmInvalid(AKM_ERR_INVALID, "x and b must be device vectors")
+    err = _d()
+    its = _i()
+    _check(load_library().akm_gmres_euler_noalloc(mat._h, float(dt), px, pb, int(restart),
+                                                  int(max_iter), float(tol), ctypes.byref(err),
+                                                  ctypes.byref(its)), allow_not_converged=True)
     return GmresInfo(err.value, its.value)
 
 

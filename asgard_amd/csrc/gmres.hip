@@ -1145,6 +1145,119 @@ extern "C" akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const 
   return rc;
 }
 
+// ------------------------------------------------- allocation-free solve ---
+
+namespace akm
+{
+//! Device memory of akm_gmres_euler_noalloc: the Krylov basis, the small
+//! work array and the step state of one solve.  Every GPU the library runs on
+//! holds its own copy until the process ends.  Host code never passes this
+//! name to a kernel or a copy: it resolves the array's device address once
+//! per GPU (noalloc_memory) and uses that.
+static __device__ __align__(256) unsigned char scratch[AKM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int noalloc_max_devices = 64;
+
+//! Per GPU: the device address of `scratch` and the host copies of the step
+//! state (plain host memory: the host waits for the stream before reading).
+struct noalloc_slot
+{
+  void *dev = nullptr;
+  alignas(64) unsigned char host[gm_slots * sizeof(gm_state<double>)];
+};
+static noalloc_slot noalloc_slots[noalloc_max_devices];
+static std::mutex noalloc_mtx;
+
+static int64_t round_up_256(int64_t b) { return (b + 255) / 256 * 256; }
+
+//! Scratch bytes of a solve of n unknowns with `restart` of `esize` bytes.
+static int64_t noalloc_bytes(int64_t n, int restart, size_t esize)
+{
+  return round_up_256(n * (restart + 1) * int64_t(esize)) +
+         round_up_256(gm_work_values(n, restart) * int64_t(esize)) +
+         round_up_256(int64_t(sizeof(gm_state<double>)));
+}
+
+//! The solve's memory carved from the current GPU's scratch array, resolving
+//! its address on the first call on that GPU.  No events: the host waits for
+//! the whole stream after every step and no step is enqueued ahead.
+static int noalloc_memory(int64_t n, int restart, size_t esize, gm_memory &mem)
+{
+  int dev = 0;
+  AKM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= noalloc_max_devices)
+    return set_error(AKM_ERR_UNSUPPORTED, "gmres noalloc: device %d above %d", dev,
+                     noalloc_max_devices - 1);
+  noalloc_slot &slot = noalloc_slots[dev];
+  {
+    std::lock_guard<std::mutex> lock(noalloc_mtx);
+    if (slot.dev == nullptr)
+    {
+      void *p = nullptr;
+      AKM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (p == nullptr)
+        return set_error(AKM_ERR_HIP, "gmres noalloc: no device address for the scratch array");
+      slot.dev = p;
+    }
+  }
+  unsigned char *base = static_cast<unsigned char *>(slot.dev);
+  const int64_t vbytes = round_up_256(n * (restart + 1) * int64_t(esize));
+  const int64_t wbytes = round_up_256(gm_work_values(n, restart) * int64_t(esize));
+  mem.V = base;
+  mem.work = base + vbytes;
+  mem.state = base + vbytes + wbytes;
+  mem.host_state = slot.host;
+  mem.ready = nullptr;
+  return AKM_SUCCESS;
+}
+} // namespace akm
+
+extern "C" akm_status akm_gmres_euler_noalloc(akm_matrix mat, double dt, void *x, const void *b,
+                                              int restart, int max_iter, double tol, double *err,
+                                              int *iterations)
+{
+  if (mat == nullptr || x == nullptr || b == nullptr)
+    return set_error(AKM_ERR_INVALID, "gmres noalloc: null argument");
+  if (mat->rows != mat->cols)
+    return set_error(AKM_ERR_INVALID, "gmres noalloc: the operator must be square");
+  const int64_t n = mat->output_size();
+  if (n <= 0)
+    return set_error(AKM_ERR_INVALID, "gmres noalloc: empty system");
+  const int64_t cap = AKM_NOALLOC_SCRATCH_BYTES;
+  if (tol <= 0)
+    tol = (mat->prec == AKM_F64) ? 1e-12 : 1e-6;
+  if (restart <= 0)
+  {
+    // the reference default, lowered until the basis fits the scratch array
+    restart = default_restarts(n, mat->esize);
+    while (restart > 1 && noalloc_bytes(n, restart, mat->esize) > cap)
+      --restart;
+  }
+  if (restart > n)
+    return set_error(AKM_ERR_INVALID, "Number of inner iterations %d must be less than %lld!",
+                     restart, static_cast<long long>(n));
+  if (restart > gm_max_restart)
+    return set_error(AKM_ERR_UNSUPPORTED, "gmres: restart %d above %d", restart, gm_max_restart);
+  const int64_t need = noalloc_bytes(n, restart, mat->esize);
+  if (need > cap)
+    return set_error(AKM_ERR_INVALID,
+                     "gmres noalloc: %lld unknowns with restart %d need %lld bytes of scratch, "
+                     "above the fixed %lld",
+                     static_cast<long long>(n), restart, static_cast<long long>(need),
+                     static_cast<long long>(cap));
+  if (max_iter <= 0)
+    max_iter = static_cast<int>(std::min<int64_t>(n, 1 << 30));
+  gm_memory mem;
+  AKM_TRY(noalloc_memory(n, restart, mat->esize, mem));
+  if (mat->prec == AKM_F64)
+    return gmres_typed<double>(*mat, mem, dt, static_cast<double *>(x),
+                               static_cast<const double *>(b), restart, max_iter, tol, err,
+                               iterations);
+  return gmres_typed<float>(*mat, mem, static_cast<float>(dt), static_cast<float *>(x),
+                            static_cast<const float *>(b), restart, max_iter,
+                            static_cast<float>(tol), err, iterations);
+}
+
 extern "C" akm_status akm_gmres_euler_mixed(akm_matrix mat, akm_matrix mat32, akm_precond pre,
                                             double dt, double *x, const double *b, int restart,
                                             int max_cycles, double tol, double inner_tol,

@@ -247,6 +247,25 @@ akm_status akm_gmres_euler(akm_matrix mat, double dt, void *x, const void *b, in
                            int max_iter, double tol, int on_device, double *err,
                            int *iterations);
 
+/* Device scratch of akm_gmres_euler_noalloc on each GPU, in bytes. */
+#define AKM_NOALLOC_SCRATCH_BYTES (32ll << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): akm_gmres_euler on device vectors x and b, with the
+ * same algorithm, results, defaults and errors, that allocates nothing.  The
+ * Krylov basis, the small work array and the step state live in a fixed
+ * per-GPU device array of AKM_NOALLOC_SCRATCH_BYTES, which each GPU holds
+ * until the process ends: about 8*(n*(restart+1) + restart^2/2) bytes for
+ * FP64 must fit, else AKM_ERR_INVALID names the need and the limit
+ * (restart <= 0 takes the reference default, lowered until it fits).  Every
+ * call on one GPU uses that GPU's array, so calls that overlap on one GPU
+ * (several threads or streams) race; each call writes every value it reads.
+ * The host waits for each Arnoldi step (no step is enqueued ahead). */
+akm_status akm_gmres_euler_noalloc(akm_matrix mat, double dt, void *x, const void *b,
+                                   int restart, int max_iter, double tol, double *err,
+                                   int *iterations);
+
 /* solver::simple_gmres(fk::matrix A, fk::vector x, fk::vector b, fk::matrix M,
  * restart, max_iter, tol) (src/solver.hpp:17-22, src/solver.cpp:54-74): the
  * reference's node-local GMRES on an explicit n x n matrix, with the same
