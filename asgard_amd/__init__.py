@@ -423,7 +423,7 @@ class KronmultMatrix:
 
     def set_kernel_family(self, family):
         """0 = register / LDS-table kernels, 1 = sum-factorised matrix-core
-        kernel (FP64 sparse-grid, d = 3, n = 3)."""
+        kernels (FP64 sparse-grid; (d, n) = (3, 3), (4, 2), (6, 2))."""
         _check(load_library().akm_set_kernel_family(self._h, int(family)))
 
     def autotune(self, x, y):

@@ -75,6 +75,7 @@ struct kron_args
   int nconn;                        // connect_1d connections
   int nnp;                          // padded n*n block size in vAk
   int64_t max_table_elems;          // largest per-row factor table (elements)
+  int64_t max_table_last4;          // the same for the last four dimensions only
   int lds_cap;                      // LDS bytes a workgroup may use (0: no tables)
   // rows in dispatch order (longest first)
   const int *row_order;

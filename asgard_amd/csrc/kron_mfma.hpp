@@ -222,7 +222,8 @@ void launch_mfma_terms(kron_args<double> const &a, int num_rows, int waves, hipS
   {
   case 1: kron_mfma_d3n3<1, TT><<<num_rows, 1 * wave_size, 0, stream>>>(a); break;
   case 2: kron_mfma_d3n3<2, TT><<<num_rows, 2 * wave_size, 0, stream>>>(a); break;
-  case 8: kron_mfma_d3n3<8, TT><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
+  case 8:
+  case 16: kron_mfma_d3n3<8, TT><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
   default: kron_mfma_d3n3<4, TT><<<num_rows, 4 * wave_size, 0, stream>>>(a); break;
   }
 }
@@ -230,236 +231,383 @@ void launch_mfma_terms(kron_args<double> const &a, int num_rows, int waves, hipS
 // ---------------------------------------------------------- d = 6, n = 2 ---
 //
 // With n = 2 two dimensions together form a dense 4 x 4 factor, the
-// Kronecker product of their 2 x 2 blocks, so one 4 x 4 x 4 product contracts
-// a pair of dimensions with no padding.  One wavefront holds one 64-entry
-// tensor, entry e = 16 g + 4 m + q in lane 16 q + 4 g + m, i.e. dimensions
-// (0, 1) on the block g, (2, 3) on m and (4, 5) on q:
-//   stage 1 (dims 4, 5):  A = x (i = (i2, i3) = m, k = (i4, i5) = q),
-//                         B[k][j] = A4(j4, i4) A5(j5, i5), j = (j4, j5) = m
-//                         -> D1[i = (i2, i3) = q][j = (j4, j5) = m]
-//   stage 2 (dims 2, 3):  A[i][k] = A2(j2, i2) A3(j3, i3) (i = m, k = q),
-//                         B = D1 -> D2[(j2, j3) = q][(j4, j5) = m]
-//   stage 3 (dims 0, 1):  lane (q, g' = (j0, j1), m) sums
-//                         A0(j0, i0) A1(j1, i1) D2 over the blocks g = (i0, i1).
-// The row's columns are in lexicographic order of (c0, ..., c5): stage 1 runs
-// per column and its results are summed over the columns sharing
-// (c0, c1, c2, c3); stage 2 runs once per such group and is summed over the
-// groups sharing (c0, c1); stage 3 runs once per (c0, c1) group.  The
-// dimension-4 block is reloaded only when (c0, ..., c4) changes.
+// Kronecker product of their 2 x 2 blocks: K45_t = A_t4 x A_t5 and so on.
+// The row's columns are in lexicographic order of (c0, ..., c5), so per term
+//   y = sum over (c0,c1) groups of K01 (sum over (c2,c3) groups of
+//       K23 (sum over the group's columns of K45 x_c)),
+// and three matrix-core layouts chain without moving data:
+//
+// stage 1, per column, v_mfma_f64_16x16x4_f64 (A 16 x 4 in lane i + 16 k,
+//   B 4 x 16 in lane j + 16 k, D 16 x 16 with D[i][j] in lane j + 16 (i % 4),
+//   item i / 4): A = the column's x with rows i = (i0,i1,i2,i3) and
+//   k = (i4,i5); B = K45 of four terms side by side, j = (t', j4, j5).  The
+//   16 x 16 result is summed over the columns sharing (c0, ..., c3); lane
+//   (hi = (i2,i3), t', (j4,j5)) holds the four (i0,i1) items.
+// stage 2, once per (c0..c3) group, v_mfma_f64_4x4x4_4b_f64 with the block
+//   index g = t' (A[i][k] in lane 16 k + 4 g + i, B[k][j] in lane 16 k + 4 g
+//   + j, D[i][j] in lane 16 i + 4 g + j): per item, D2 = K23_t D1, whose B
+//   operand is the stage-1 item exactly where it sits.  Summed over the
+//   groups sharing (c0, c1) in the accumulator; lane (hi = (j2,j3), t',
+//   (j4,j5)).
+// stage 3, once per (c0,c1) group, in each lane: the four (i0,i1) items
+//   times K01_t as two 2 x 2 passes (VALU), summed over the groups.
+// At the end the four terms of a lane quad-row are summed by a fixed xor
+// tree, then the blocks of four terms, then the wavefronts in LDS in
+// wavefront order.
+//
+// The factors of dimensions 2-5 change every column or every few columns;
+// the workgroup stages them in LDS (every connect_1d connection of the row's
+// 1D cells in those dimensions, in the [dim][connection][term][4] layout of
+// vAk) with a table from a 1D index to its LDS offset, so the only per-column
+// global loads are the x slices, XD columns of them in flight while a column
+// is contracted.  Dimensions 0 and 1 change once per (c0,c1) group (about 30
+// columns at d=6, |l|_1 <= 7): their blocks are read from global memory when
+// the group opens and used when it closes.
 
-//! Element of a 2 x 2 column-major block pair product for lane (q, m):
-//! A(jhi, ihi) B(jlo, ilo) with (ihi, ilo) = bits of q, (jhi, jlo) = bits of m.
-__device__ __forceinline__ int pair_hi(int q, int m) { return (q >> 1) * 2 + (m >> 1); }
-__device__ __forceinline__ int pair_lo(int q, int m) { return (q & 1) * 2 + (m & 1); }
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
-template<int W, int TT>
+__device__ __forceinline__ mfma_d4 mfma16(double a, double b, mfma_d4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+
+//! LDS bytes of kron_mfma_d6n2: 1D-index tables of the six dimensions, the
+//! row's dimension-2..5 factor tables, and the wavefronts' row sums.
+inline size_t mfma_d6n2_lds_bytes(kron_args<double> const &a, int W)
+{
+  const size_t locb = ((size_t(6) * a.M * sizeof(int) + 15) / 16) * 16;
+  const size_t tab = ((size_t(a.max_table_last4) * sizeof(double) + 15) / 16) * 16;
+  return locb + tab + size_t(W) * wave_size * sizeof(double);
+}
+
+template<int W, int TG, int XD>
 __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double> a)
 {
-  constexpr int N = 64;
-  __shared__ double red[W * wave_size];
-  const int lane = threadIdx.x & (wave_size - 1);
-  const int wave = threadIdx.x / wave_size;
-  const int q = lane >> 4, g = (lane >> 2) & 3, m = lane & 3;
+  constexpr int N = 64, nnp = 4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int M = a.M;
   const int T = a.T;
+  const int64_t tstride = int64_t(T) * nnp; // one connection in vAk
+  int *loc = reinterpret_cast<int *>(smem_raw);
+  double *fac = reinterpret_cast<double *>(smem_raw + ((6 * M * sizeof(int) + 15) / 16) * 16);
+  double *red = fac + ((a.max_table_last4 + 1) / 2) * 2;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (wave_size - 1);
+  const int wave = tid / wave_size;
+  const int lo = lane & 15, hi = lane >> 4, tq = lo >> 2;
 
   const int row = a.row_order[blockIdx.x];
   const int64_t jbeg = a.pntr[row], jend = a.pntr[row + 1];
   const int64_t len = jend - jbeg;
   const int64_t cb = jbeg + (len * wave) / W, ce = jbeg + (len * (wave + 1)) / W;
 
-  int rM[6];
+  int rr[6];
 #pragma unroll
   for (int k = 0; k < 6; k++)
-    rM[k] = a.elem[int64_t(row) * 6 + k] * a.M;
-  const int64_t blk = int64_t(T) * 24; // one connection: T terms x 6 dims x 4
-  const int xoff = 16 * g + 4 * m + q;
-  const int ehi = pair_hi(q, m), elo = pair_lo(q, m);
-
-  double s1[TT], s2[TT], a4[TT], k23[TT], k01[TT][4];
-#pragma unroll
-  for (int t = 0; t < TT; t++)
+    rr[k] = a.elem[int64_t(row) * 6 + k];
+  // dimensions 2..5: first connect_1d connection of the row's 1D cell and
+  // LDS offset of its table
+  int cbk[6], fbk[6];
   {
-    s1[t] = s2[t] = a4[t] = k23[t] = 0;
-    k01[t][0] = k01[t][1] = k01[t][2] = k01[t][3] = 0;
+    int base = 0;
+#pragma unroll
+    for (int k = 2; k < 6; k++)
+    {
+      cbk[k] = a.conn_pntr[rr[k]];
+      fbk[k] = base;
+      base += (a.conn_pntr[rr[k] + 1] - cbk[k]) * T * nnp;
+    }
   }
-  double acc = 0;
 
-  auto block_of = [&](int k, int c) -> int64_t {
-    return int64_t(a.off1d[rM[k] + c]) * blk + k * 4;
-  };
-  auto load_a4 = [&](int c4) {
-    const int64_t b4 = block_of(4, c4);
+  // loc[k][c]: dimensions 0, 1 the connect_1d position of (rr[k], c);
+  // dimensions 2..5 the LDS offset of its term-0 block (0 if unconnected)
+  for (int e = tid; e < 6 * M; e += W * wave_size)
+  {
+    const int k = e / M, c = e - k * M;
+    int v = 0;
 #pragma unroll
-    for (int t = 0; t < TT; t++)
-      if (t < T)
-        a4[t] = a.vA[b4 + t * 24 + ehi];
-  };
-  auto load_k23 = [&](int c2, int c3) {
-    const int64_t b2 = block_of(2, c2), b3 = block_of(3, c3);
-#pragma unroll
-    for (int t = 0; t < TT; t++)
-      if (t < T)
-        k23[t] = a.vA[b2 + t * 24 + ehi] * a.vA[b3 + t * 24 + elo];
-  };
-  // A0(j0, i0) A1(j1, i1) for output block g' = (j0, j1) and each source
-  // block (i0, i1)
-  auto load_k01 = [&](int c0, int c1) {
-    const int64_t b0 = block_of(0, c0), b1 = block_of(1, c1);
-    const int j0 = g >> 1, j1 = g & 1;
-#pragma unroll
-    for (int t = 0; t < TT; t++)
-      if (t < T)
+    for (int q = 0; q < 6; q++)
+      if (q == k)
       {
-        const double p00 = a.vA[b0 + t * 24 + 0 * 2 + j0], p01 = a.vA[b0 + t * 24 + 1 * 2 + j0];
-        const double p10 = a.vA[b1 + t * 24 + 0 * 2 + j1], p11 = a.vA[b1 + t * 24 + 1 * 2 + j1];
-        k01[t][0] = p00 * p10; // (i0, i1) = (0, 0)
-        k01[t][1] = p00 * p11; // (0, 1)
-        k01[t][2] = p01 * p10; // (1, 0)
-        k01[t][3] = p01 * p11; // (1, 1)
+        const int off = a.off1d[rr[q] * M + c];
+        if (q < 2)
+          v = off;
+        else
+          v = (off < 0) ? 0 : (off - cbk[q]) * T * nnp + fbk[q];
       }
+    loc[e] = v;
+  }
+#pragma unroll
+  for (int k = 2; k < 6; k++)
+  {
+    const int len = (a.conn_pntr[rr[k] + 1] - cbk[k]) * T * nnp;
+    const double *src = a.vAk + (int64_t(k) * a.nconn + cbk[k]) * tstride;
+    for (int e = tid; e < len; e += W * wave_size)
+      fac[fbk[k] + e] = src[e];
+  }
+  __syncthreads();
+
+  // lane roles
+  const int xoff = (lo << 2) | hi; // x[(i0,i1,i2,i3) = lo, (i4,i5) = hi]
+  const int ehi = ((hi >> 1) << 1) | ((lo >> 1) & 1);        // dims 4 / 2 block entry
+  const int elo = ((hi & 1) << 1) | (lo & 1);                // dims 5 / 3 block entry
+  int tcl[TG];
+  bool tok[TG];
+#pragma unroll
+  for (int b = 0; b < TG; b++)
+  {
+    tok[b] = (4 * b + tq) < T;
+    tcl[b] = tok[b] ? 4 * b + tq : 0;
+  }
+
+  mfma_d4 d1[TG], acc2[TG];
+  double acc3[TG][4];
+  double a2v[TG], a3v[TG]; // dims 2, 3 block entries of the open (c0..c3) group
+  int q0 = 0, q1 = 0;      // connect_1d positions of the open (c0,c1) group
+#pragma unroll
+  for (int b = 0; b < TG; b++)
+  {
+    d1[b] = mfma_d4{0, 0, 0, 0};
+    acc2[b] = mfma_d4{0, 0, 0, 0};
+    a2v[b] = a3v[b] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      acc3[b][q] = 0;
+  }
+
+  const double *__restrict__ vAk = a.vAk;
+  auto block_of = [&](int k, int conn, int t) -> const double * {
+    return vAk + (int64_t(k) * a.nconn + conn) * tstride + int64_t(t) * nnp;
   };
-  auto close_4 = [&]() {
+  // the group's dims 2, 3 entries are read from LDS when it opens and used
+  // when it closes; lanes of terms past T read term 0 (their sums are zero)
+  auto open_23 = [&](int c2, int c3) {
+    const int l2 = loc[2 * M + c2], l3 = loc[3 * M + c3];
 #pragma unroll
-    for (int t = 0; t < TT; t++)
-      if (t < T)
-      {
-        s2[t] = mfma4(k23[t], s1[t], s2[t]);
-        s1[t] = 0;
-      }
+    for (int b = 0; b < TG; b++)
+    {
+      a2v[b] = fac[l2 + tcl[b] * nnp + ehi];
+      a3v[b] = fac[l3 + tcl[b] * nnp + elo];
+    }
   };
-  auto close_2 = [&]() {
-    const int src = 16 * q + m;
+  auto open_01 = [&](int c0, int c1) {
+    q0 = loc[c0];
+    q1 = loc[M + c1];
+  };
+  // the open (c0..c3) group: its stage-1 sums through K23 into acc2
+  auto close_23 = [&]() {
 #pragma unroll
-    for (int t = 0; t < TT; t++)
-      if (t < T)
+    for (int b = 0; b < TG; b++)
+    {
+      const double k23 = a2v[b] * a3v[b];
+      acc2[b][0] = mfma4(k23, d1[b][0], acc2[b][0]);
+      acc2[b][1] = mfma4(k23, d1[b][1], acc2[b][1]);
+      acc2[b][2] = mfma4(k23, d1[b][2], acc2[b][2]);
+      acc2[b][3] = mfma4(k23, d1[b][3], acc2[b][3]);
+      d1[b] = mfma_d4{0, 0, 0, 0};
+    }
+  };
+  // the open (c0,c1) group: items (i0,i1) through K01 into acc3[(j0,j1)];
+  // A(j, i) is entry i * 2 + j of a column-major 2 x 2 block
+  auto close_01 = [&]() {
+#pragma unroll
+    for (int b = 0; b < TG; b++)
+    {
+      // dims 0, 1 blocks, read from global memory once per (c0,c1) group
+      double k0r[4], k1r[4];
+      const double *p0 = block_of(0, q0, tcl[b]);
+      const double *p1 = block_of(1, q1, tcl[b]);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
       {
-        double v = acc;
-#pragma unroll
-        for (int gs = 0; gs < 4; gs++)
-          v = fma(k01[t][gs], __shfl(s2[t], src + 4 * gs, wave_size), v);
-        acc = v;
-        s2[t] = 0;
+        k0r[q] = p0[q];
+        k1r[q] = p1[q];
       }
+      double u[2][2]; // [j0][i1]
+#pragma unroll
+      for (int j0 = 0; j0 < 2; j0++)
+#pragma unroll
+        for (int i1 = 0; i1 < 2; i1++)
+          u[j0][i1] = fma(k0r[2 + j0], acc2[b][2 + i1], k0r[j0] * acc2[b][i1]);
+#pragma unroll
+      for (int j0 = 0; j0 < 2; j0++)
+#pragma unroll
+        for (int j1 = 0; j1 < 2; j1++)
+          acc3[b][2 * j0 + j1] += fma(k1r[2 + j1], u[j0][1], k1r[j1] * u[j0][0]);
+      acc2[b] = mfma_d4{0, 0, 0, 0};
+    }
   };
 
-  constexpr unsigned long long mask2 = (1ull << 20) - 1, mask4 = (1ull << 40) - 1,
-                               mask5 = (1ull << 50) - 1, none = ~0ull;
-  unsigned long long prev2 = none, prev4 = none, prev5 = none;
+  constexpr unsigned long long mask01 = (1ull << 20) - 1, mask03 = (1ull << 40) - 1, none = ~0ull;
+  unsigned long long prev01 = none, prev03 = none;
   for (int64_t j0 = cb; j0 < ce; j0 += wave_size)
   {
+    // one batch of up to 64 columns, one per lane: id, packed 1D indices,
+    // LDS offsets of the dimension-4 and -5 blocks
     const int64_t jl = j0 + lane;
-    int colv = 0, off5v = 0;
+    int colv = 0, l4v = 0, l5v = 0;
     unsigned long long wv = 0;
     if (jl < ce)
     {
       colv = a.indx[jl];
       wv = a.epack[colv];
-      off5v = a.off1d[rM[5] + packed_1d(wv, 5)];
+      l4v = loc[4 * M + packed_1d(wv, 4)];
+      l5v = loc[5 * M + packed_1d(wv, 5)];
     }
     const unsigned int wlo = static_cast<unsigned int>(wv), whi = static_cast<unsigned int>(wv >> 32);
     const int nb = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
-    double xn = 0, fn[TT];
-    auto fetch = [&](int jj, double &xv, double (&fv)[TT]) {
-      const int col = __builtin_amdgcn_readlane(colv, jj);
-      const int64_t b5 = int64_t(__builtin_amdgcn_readlane(off5v, jj)) * blk + 20;
-      xv = a.x[int64_t(col) * N + xoff];
+    double xq[XD];
 #pragma unroll
-      for (int t = 0; t < TT; t++)
-        if (t < T)
-          fv[t] = a.vA[b5 + t * 24 + elo];
-    };
-    fetch(0, xn, fn);
-    for (int jj = 0; jj < nb; jj++)
+    for (int u = 0; u < XD; u++)
     {
-      const double xv = xn;
-      double f5[TT];
+      const int col = __builtin_amdgcn_readlane(colv, u < nb ? u : 0);
+      xq[u] = a.x[int64_t(col) * N + xoff];
+    }
+    // dimension-4/5 entries of the next column, read one column ahead
+    double f4n[TG], f5n[TG];
+    auto fetch_factors = [&](int jj) {
+      const int l4 = __builtin_amdgcn_readlane(l4v, jj);
+      const int l5 = __builtin_amdgcn_readlane(l5v, jj);
 #pragma unroll
-      for (int t = 0; t < TT; t++)
-        f5[t] = fn[t];
-      if (jj + 1 < nb)
-        fetch(jj + 1, xn, fn);
-      const unsigned long long w =
-          (static_cast<unsigned long long>(
-               static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
-           << 32) |
-          static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
-      const unsigned long long p5 = w & mask5;
-      if (p5 != prev5)
+      for (int b = 0; b < TG; b++)
       {
-        const unsigned long long p4 = w & mask4;
-        if (p4 != prev4)
-        {
-          if (prev4 != none)
-            close_4();
-          const unsigned long long p2 = w & mask2;
-          if (p2 != prev2)
-          {
-            if (prev2 != none)
-              close_2();
-            load_k01(packed_1d(w, 0), packed_1d(w, 1));
-            prev2 = p2;
-          }
-          load_k23(packed_1d(w, 2), packed_1d(w, 3));
-          prev4 = p4;
-        }
-        load_a4(packed_1d(w, 4));
-        prev5 = p5;
+        f4n[b] = fac[l4 + tcl[b] * nnp + ehi];
+        f5n[b] = fac[l5 + tcl[b] * nnp + elo];
       }
+    };
+    fetch_factors(0);
+    for (int jb = 0; jb < nb; jb += XD)
+    {
 #pragma unroll
-      for (int t = 0; t < TT; t++)
-        if (t < T)
-          s1[t] = mfma4(xv, a4[t] * f5[t], s1[t]);
+      for (int u = 0; u < XD; u++)
+      {
+        const int jj = jb + u;
+        if (jj < nb)
+        {
+          const double xv = xq[u];
+          double bv[TG];
+#pragma unroll
+          for (int b = 0; b < TG; b++)
+            bv[b] = tok[b] ? f4n[b] * f5n[b] : 0.0;
+          fetch_factors((jj + 1 < nb) ? jj + 1 : jj);
+          {
+            // refill unconditionally (the tail re-reads the batch's last
+            // column) so that no branch joins an x load in flight
+            const int nxt = (jj + XD < nb) ? jj + XD : nb - 1;
+            const int col = __builtin_amdgcn_readlane(colv, nxt);
+            xq[u] = a.x[int64_t(col) * N + xoff];
+          }
+          const unsigned long long w =
+              (static_cast<unsigned long long>(
+                   static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
+               << 32) |
+              static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
+          const unsigned long long p03 = w & mask03;
+          if (p03 != prev03)
+          {
+            if (prev03 != none)
+              close_23();
+            const unsigned long long p01 = w & mask01;
+            if (p01 != prev01)
+            {
+              if (prev01 != none)
+                close_01();
+              open_01(packed_1d(w, 0), packed_1d(w, 1));
+              prev01 = p01;
+            }
+            open_23(packed_1d(w, 2), packed_1d(w, 3));
+            prev03 = p03;
+          }
+#pragma unroll
+          for (int b = 0; b < TG; b++)
+            d1[b] = mfma16(xv, bv[b], d1[b]);
+        }
+      }
     }
   }
-  if (prev4 != none)
+  if (prev03 != none)
   {
-    close_4();
-    close_2();
+    close_23();
+    close_01();
   }
 
-  // lane (q = (j2, j3), g = (j0, j1), m = (j4, j5)) holds entry 16 g + 4 q + m
-  red[wave * wave_size + lane] = acc;
+  // terms: blocks in order, then the four t' of a lane row by a fixed tree;
+  // lanes with t' = 0 hold entry (j0,j1, j2,j3 = hi, j4,j5 = lane & 3)
+  double v[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+  {
+    double s = acc3[0][q];
+#pragma unroll
+    for (int b = 1; b < TG; b++)
+      s += acc3[b][q];
+    s += __shfl_xor(s, 4, wave_size);
+    s += __shfl_xor(s, 8, wave_size);
+    v[q] = s;
+  }
+  if (tq == 0)
+  {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      red[wave * wave_size + q * 16 + hi * 4 + (lane & 3)] = v[q];
+  }
   __syncthreads();
   const int64_t ybase = int64_t(row) * N;
-  for (int e = threadIdx.x; e < N; e += W * wave_size)
+  for (int e = tid; e < N; e += W * wave_size)
   {
-    const int src = 16 * ((e >> 2) & 3) + 4 * (e >> 4) + (e & 3);
-    double tot = red[src];
+    double tot = red[e];
     for (int w = 1; w < W; w++)
-      tot += red[w * wave_size + src];
+      tot += red[w * wave_size + e];
     store_row(a, ybase + e, tot);
   }
 }
 
-template<int TT>
-void launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
+template<int TG>
+bool launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
 {
+  auto go = [&](auto kern, int W) -> bool {
+    const size_t lds = mfma_d6n2_lds_bytes(a, W);
+    if (lds > 160 * 1024)
+      return false;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      return false;
+    }
+    kern<<<num_rows, W * wave_size, lds, stream>>>(a);
+    return true;
+  };
   switch (waves)
   {
-  case 1: kron_mfma_d6n2<1, TT><<<num_rows, 1 * wave_size, 0, stream>>>(a); break;
-  case 2: kron_mfma_d6n2<2, TT><<<num_rows, 2 * wave_size, 0, stream>>>(a); break;
-  case 8: kron_mfma_d6n2<8, TT><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
-  default: kron_mfma_d6n2<4, TT><<<num_rows, 4 * wave_size, 0, stream>>>(a); break;
+  // XD x loads in flight per wavefront; 16 wavefronts keep 4 to fit 128 VGPRs
+  case 1: return go(kron_mfma_d6n2<1, TG, 8>, 1);
+  case 2: return go(kron_mfma_d6n2<2, TG, 8>, 2);
+  case 8: return go(kron_mfma_d6n2<8, TG, 8>, 8);
+  case 16: return go(kron_mfma_d6n2<16, TG, 4>, 16);
+  default: return go(kron_mfma_d6n2<4, TG, 8>, 4);
   }
 }
 
 inline bool launch_mfma_d6n2(kron_args<double> const &a, int num_rows, int waves,
                              hipStream_t stream)
 {
-  if (a.T > mfma_max_terms || a.epack == nullptr)
+  if (a.T > mfma_max_terms || a.epack == nullptr || a.vAk == nullptr || a.M > 1024)
     return false;
-  if (a.T <= 2)
-    launch_mfma6_terms<2>(a, num_rows, waves, stream);
-  else if (a.T <= 4)
-    launch_mfma6_terms<4>(a, num_rows, waves, stream);
-  else if (a.T <= 6)
-    launch_mfma6_terms<6>(a, num_rows, waves, stream);
-  else
-    launch_mfma6_terms<mfma_max_terms>(a, num_rows, waves, stream);
-  return true;
+  if (a.T <= 4)
+    return launch_mfma6_terms<1>(a, num_rows, waves, stream);
+  return launch_mfma6_terms<2>(a, num_rows, waves, stream);
 }
-
 // ---------------------------------------------------------- d = 4, n = 2 ---
+
+//! Element of a 2 x 2 column-major block pair product for lane (q, m):
+//! A(jhi, ihi) B(jlo, ilo) with (ihi, ilo) = bits of q, (jhi, jlo) = bits of m.
+__device__ __forceinline__ int pair_hi(int q, int m) { return (q >> 1) * 2 + (m >> 1); }
+__device__ __forceinline__ int pair_lo(int q, int m) { return (q & 1) * 2 + (m & 1); }
 //
 // A 16-entry tensor fills one block of the 4 x 4 x 4 product, so the four
 // blocks carry four terms of the same column at once (term 4 p + g in block
@@ -619,7 +767,8 @@ void launch_mfma4_terms(kron_args<double> const &a, int num_rows, int waves, hip
   {
   case 1: kron_mfma_d4n2<1, TG><<<num_rows, 1 * wave_size, 0, stream>>>(a); break;
   case 2: kron_mfma_d4n2<2, TG><<<num_rows, 2 * wave_size, 0, stream>>>(a); break;
-  case 8: kron_mfma_d4n2<8, TG><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
+  case 8:
+  case 16: kron_mfma_d4n2<8, TG><<<num_rows, 8 * wave_size, 0, stream>>>(a); break;
   default: kron_mfma_d4n2<4, TG><<<num_rows, 4 * wave_size, 0, stream>>>(a); break;
   }
 }

@@ -45,8 +45,9 @@ static int ipow_int(int n, int d)
 
 int waves_per_row(akm_matrix_s const &m)
 {
+  // 16 wavefronts per row exist only for the matrix-core kernels
   if (m.waves > 0)
-    return m.waves;
+    return (m.waves > 8 && m.family != 1) ? 8 : m.waves;
   // enough wavefronts to cover the 1024 SIMDs several times over
   const int64_t target = 8 * 1024;
   int w = 1;
@@ -139,6 +140,7 @@ static kron_args<P> matrix_args(akm_matrix_s const &m)
   a.nconn = m.nconn;
   a.nnp = m.nnp;
   a.max_table_elems = m.max_table_elems;
+  a.max_table_last4 = m.max_table_last4;
   a.lds_cap = (m.vAk.data() != nullptr) ? lds_cap_bytes(m) : 0;
   a.row_order = m.row_order.data();
   a.family = m.family;
@@ -405,18 +407,23 @@ akm_status akm_create_sparse_grid(int d, int n, int64_t num_cells, const int *ta
       AKM_TRY(m->conn_pntr.upload(p1.data(), int64_t(p1.size())));
       m->nnp = (m->prec == AKM_F64) ? padded_block<double>(n) : padded_block<float>(n);
       AKM_TRY(upload_vAk(*m, vA));
-      int64_t mx = 0;
+      int64_t mx = 0, mx2 = 0;
       for (int64_t c = 0; c < num_cells; c++)
       {
-        int64_t s = 0;
+        int64_t s = 0, s2 = 0;
         for (int k = 0; k < d; k++)
         {
           const int r = elem[c * d + k];
-          s += int64_t(p1[r + 1] - p1[r]) * T * m->nnp;
+          const int64_t len = int64_t(p1[r + 1] - p1[r]) * T * m->nnp;
+          s += len;
+          if (k >= d - 4)
+            s2 += len;
         }
         mx = std::max(mx, s);
+        mx2 = std::max(mx2, s2);
       }
       m->max_table_elems = mx;
+      m->max_table_last4 = mx2;
     }
   }
   AKM_TRY(build_schedule(*m));
@@ -543,8 +550,8 @@ akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
   if (mat->mode == mode_global)
     return global_set_layout(*mat, waves_per_row);
   if (waves_per_row != 0 && waves_per_row != 1 && waves_per_row != 2 && waves_per_row != 4 &&
-      waves_per_row != 8)
-    return set_error(AKM_ERR_INVALID, "waves per row must be 0, 1, 2, 4 or 8 (got %d)",
+      waves_per_row != 8 && waves_per_row != 16)
+    return set_error(AKM_ERR_INVALID, "waves per row must be 0, 1, 2, 4, 8 or 16 (got %d)",
                      waves_per_row);
   if (lds_cap_bytes > 160 * 1024)
     return set_error(AKM_ERR_INVALID, "LDS cap %d above the 160 KB of a CU", lds_cap_bytes);
@@ -639,7 +646,7 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
     {
       const int saved_lds = mat->lds_cap;
       mat->family = 1;
-      for (int w : {1, 2, 4, 8})
+      for (int w : {1, 2, 4, 8, 16})
       {
         mat->waves = w;
         double ms = 0;

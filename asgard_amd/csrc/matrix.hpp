@@ -36,6 +36,7 @@ struct akm_matrix_s
   std::vector<int> conn_pntr_host;
   int nconn = 0, nnp = 0;
   int64_t max_table_elems = 0;
+  int64_t max_table_last4 = 0; // largest per-row table of the last four dimensions
   int lds_cap = -1; // -1 automatic, 0 disables the LDS tables
   // dense mode
   std::vector<akm::dev_buffer<unsigned char>> terms;

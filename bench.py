@@ -27,6 +27,9 @@ import time
 # OpenMP library loads
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# the CPUs this job may run on, read before an OpenMP runtime binds the
+# initial thread to its first place (after which the mask holds one CPU)
+JOB_CPUS = len(os.sched_getaffinity(0))
 
 import numpy as np
 
@@ -62,12 +65,12 @@ def cpu_threads():
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         return int(env)
-    return len(os.sched_getaffinity(0))
+    return JOB_CPUS
 
 
 def cpu_host_info(threads):
     return {"cores": threads, "host_logical_cpus": os.cpu_count(),
-            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "affinity_cpus": JOB_CPUS,
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "omp_places": os.environ.get("OMP_PLACES")}
 
@@ -169,6 +172,18 @@ def headline_workload(seed=0):
     vA = grid.build_vA(d, n, T, L, ops)
     x = np.random.default_rng(seed).uniform(-1.0, 1.0, table.shape[0] * n ** d)
     return d, n, L, T, table, vA, x
+
+
+def choose_shape(mat, xd, yd, shape):
+    """The kernel shape of an apply: the autotuner's choice, or `shape` =
+    (waves per row, kernel) with kernel 0 = factors from global memory, 1 =
+    LDS factor tables, 2 = matrix cores (for reproducible profiles)."""
+    if shape is None:
+        return mat.autotune(xd, yd)
+    waves, kernel = shape
+    mat.set_kernel_family(1 if kernel == 2 else 0)
+    mat.set_tuning(waves_per_row=waves, lds_cap_bytes=-1 if kernel == 1 else 0)
+    return waves, kernel
 
 
 def time_steps(mat, xd, yd, steps, warmup, stream):
@@ -573,7 +588,7 @@ def calibrate_copy(mib=512, reps=5):
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}))
 
 
-def small_apply(stream, no_cpu, steps=100, warmup=10):
+def small_apply(stream, no_cpu, steps=100, warmup=10, shape=None):
     """The d=3 n=3 |l|_1<=6 apply with 3 random terms: its GFLOP/s, roofline
     and parity against the oracle's full apply, and its DG / FP32 variants."""
     import torch
@@ -586,7 +601,7 @@ def small_apply(stream, no_cpu, steps=100, warmup=10):
     mat.set_stream(stream.cuda_stream)
     xd = torch.from_numpy(x).cuda()
     yd = torch.zeros_like(xd)
-    waves, lds = mat.autotune(xd, yd)
+    waves, lds = choose_shape(mat, xd, yd, shape)
     ms, _ = time_steps(mat, xd, yd, steps, warmup, stream)
     nconn = len(asgard_amd.connect_1d(L)[1])
     rec = {"workload": SMALL["workload"], "cells": int(table.shape[0]),
@@ -682,7 +697,12 @@ def main():
                     help="print only the global-operator applies (one JSON line)")
     ap.add_argument("--samples", type=int, default=200_000,
                     help="per-cell training samples of the learned preconditioner")
+    ap.add_argument("--shape", default=None,
+                    help="W:K pins the kronmult kernel shape instead of autotuning: W "
+                         "wavefronts per row, K = 0 factors from global memory, 1 LDS "
+                         "factor tables, 2 matrix cores (for profiling)")
     args = ap.parse_args()
+    shape = tuple(int(v) for v in args.shape.split(":")) if args.shape else None
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
               "supported (the row-sharded multi-GPU design is described in DESIGN.md)",
@@ -712,7 +732,7 @@ def main():
 
     stream = torch.cuda.current_stream()
     if args.small_only:
-        print(json.dumps(small_apply(stream, args.no_cpu)), flush=True)
+        print(json.dumps(small_apply(stream, args.no_cpu, shape=shape)), flush=True)
         return
 
     d, n, L, T, table, vA, x = headline_workload()
@@ -725,7 +745,7 @@ def main():
     nnz = mat.num_pairs()
     nconn = len(asgard_amd.connect_1d(L)[1])
     # setup, untimed: the autotuner picks the workgroup shape and LDS layout
-    waves, lds_tables = mat.autotune(xd, yd)
+    waves, lds_tables = choose_shape(mat, xd, yd, shape)
     ms_per_step, wall = time_steps(mat, xd, yd, args.steps, args.warmup, stream)
     gflops = flops / (ms_per_step * 1e-3) * 1e-9
 

@@ -182,8 +182,8 @@ akm_status akm_apply_euler(akm_matrix mat, double dt, double alpha, const void *
 akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_size,
                            int64_t *output_size, int64_t *num_pairs, int64_t *flops);
 
-/* Tuning knobs: wavefronts per row workgroup (0 = automatic, else 1, 2, 4 or
- * 8), the LDS bytes a workgroup may hold for the row's factor tables
+/* Tuning knobs: wavefronts per row workgroup (0 = automatic, else 1, 2, 4,
+ * 8 or 16; 16 only for the matrix-core kernels, the others run 8), the LDS bytes a workgroup may hold for the row's factor tables
  * (-1 = automatic, 0 = read factors from global memory) and whether to
  * force the block-wide LDS kernel (for tests).  Takes effect at the next
  * apply.  On a global operator only waves_per_row counts and picks the row
@@ -210,13 +210,15 @@ akm_status akm_autotune(akm_matrix mat, const void *x, void *y, int *waves_per_r
                         int *lds_tables);
 
 /* Kernel family of the apply: 0 = the register / LDS-table kernels, 1 = the
- * sum-factorised matrix-core kernel for FP64 sparse-grid matrices with
- * d = 3, n = 3, 1 to 8 terms and max_level <= 10 (the row's columns taken in
- * lexicographic order of their 1D indices; dimensions 2 and 1 contracted by
- * v_mfma_f64_4x4x4_4b_f64 with the group sums in its accumulator, dimension 0
- * once per group).  AKM_ERR_UNSUPPORTED for any other matrix.  Takes effect
- * at the next apply; akm_set_tuning's waves_per_row sets its wavefronts per
- * row.  No counterpart in the reference. */
+ * sum-factorised matrix-core kernels for FP64 sparse-grid matrices with
+ * (d, n) = (3, 3), (4, 2) or (6, 2), 1 to 8 terms and max_level <= 10: the
+ * row's columns are taken in lexicographic order of their 1D indices, the
+ * trailing dimensions are contracted per column on the FP64 matrix cores and
+ * summed over the columns that share the leading 1D indices before the
+ * leading dimensions are contracted once per such group.
+ * AKM_ERR_UNSUPPORTED for any other matrix.  Takes effect at the next apply;
+ * akm_set_tuning's waves_per_row sets its wavefronts per row.  No
+ * counterpart in the reference. */
 akm_status akm_set_kernel_family(akm_matrix mat, int family);
 
 /* Times `iterations` back-to-back applies (device x/y) with HIP events on the

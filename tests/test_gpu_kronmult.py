@@ -384,10 +384,11 @@ def test_mfma_kernel_refined_grid():
 
 
 @pytest.mark.parametrize("L,T,seed", [(2, 1, 1), (3, 2, 2), (3, 5, 3), (4, 6, 4), (3, 8, 5)])
-@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8, 16])
 def test_mfma_d6n2_kernel_parity(L, T, seed, waves):
-    """d=6, n=2: dimensions paired into dense 4 x 4 Kronecker factors on
-    v_mfma_f64_4x4x4_4b_f64; against the oracle for every term bound."""
+    """d=6, n=2: dimensions paired into dense 4 x 4 Kronecker factors, four
+    terms per v_mfma_f64_16x16x4_f64 per column, then v_mfma_f64_4x4x4_4b_f64
+    per group; against the oracle for every term bound and workgroup shape."""
     c = sparse_grid_case(6, 2, L, T, seed=seed, dist="normal")
     m = asgard_amd.KronmultMatrix.sparse_grid(6, 2, c["table"], L, T, c["vA"])
     m.set_kernel_family(1)
