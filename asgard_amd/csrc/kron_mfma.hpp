@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d3n3(kron_args<double>
   constexpr int N = 27;
   __shared__ double red[W * wave_size];
   const int lane = threadIdx.x & (wave_size - 1);
-  const int wave = threadIdx.x / wave_size;
+  // uniform per wavefront: loop bounds and group keys stay scalar
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / wave_size);
   const int q = lane >> 4, g = (lane >> 2) & 3, m = lane & 3;
   const int T = a.T;
 
@@ -273,29 +274,37 @@ __device__ __forceinline__ mfma_d4 mfma16(double a, double b, mfma_d4 c)
 
 
 //! LDS bytes of kron_mfma_d6n2: 1D-index tables of the six dimensions, the
-//! row's dimension-2..5 factor tables, and the wavefronts' row sums.
-inline size_t mfma_d6n2_lds_bytes(kron_args<double> const &a, int W)
+//! row's factor tables (dimensions 0..5 with ALL6, else 2..5), and the
+//! wavefronts' row sums.
+inline size_t mfma_d6n2_lds_bytes(kron_args<double> const &a, int W, bool all6)
 {
   const size_t locb = ((size_t(6) * a.M * sizeof(int) + 15) / 16) * 16;
-  const size_t tab = ((size_t(a.max_table_last4) * sizeof(double) + 15) / 16) * 16;
+  const int64_t elems = all6 ? a.max_table_elems : a.max_table_last4;
+  const size_t tab = ((size_t(elems) * sizeof(double) + 15) / 16) * 16;
   return locb + tab + size_t(W) * wave_size * sizeof(double);
 }
 
-template<int W, int TG, int XD>
+#ifndef D6_XD8
+#define D6_XD8 16
+#endif
+
+template<int W, int TG, int XD, bool ALL6>
 __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double> a)
 {
   constexpr int N = 64, nnp = 4;
+  constexpr int K0 = ALL6 ? 0 : 2; // first dimension whose factors are staged in LDS
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int M = a.M;
   const int T = a.T;
   const int64_t tstride = int64_t(T) * nnp; // one connection in vAk
   int *loc = reinterpret_cast<int *>(smem_raw);
   double *fac = reinterpret_cast<double *>(smem_raw + ((6 * M * sizeof(int) + 15) / 16) * 16);
-  double *red = fac + ((a.max_table_last4 + 1) / 2) * 2;
+  double *red = fac + (((ALL6 ? a.max_table_elems : a.max_table_last4) + 1) / 2) * 2;
 
   const int tid = threadIdx.x;
   const int lane = tid & (wave_size - 1);
-  const int wave = tid / wave_size;
+  // uniform per wavefront: loop bounds and group keys stay scalar
+  const int wave = __builtin_amdgcn_readfirstlane(tid / wave_size);
   const int lo = lane & 15, hi = lane >> 4, tq = lo >> 2;
 
   const int row = a.row_order[blockIdx.x];
@@ -307,13 +316,13 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
 #pragma unroll
   for (int k = 0; k < 6; k++)
     rr[k] = a.elem[int64_t(row) * 6 + k];
-  // dimensions 2..5: first connect_1d connection of the row's 1D cell and
+  // staged dimensions: first connect_1d connection of the row's 1D cell and
   // LDS offset of its table
   int cbk[6], fbk[6];
   {
     int base = 0;
 #pragma unroll
-    for (int k = 2; k < 6; k++)
+    for (int k = K0; k < 6; k++)
     {
       cbk[k] = a.conn_pntr[rr[k]];
       fbk[k] = base;
@@ -321,8 +330,8 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     }
   }
 
-  // loc[k][c]: dimensions 0, 1 the connect_1d position of (rr[k], c);
-  // dimensions 2..5 the LDS offset of its term-0 block (0 if unconnected)
+  // loc[k][c]: for a dimension staged in LDS the LDS offset of the term-0
+  // block of (rr[k], c) (0 if unconnected), otherwise its connect_1d position
   for (int e = tid; e < 6 * M; e += W * wave_size)
   {
     const int k = e / M, c = e - k * M;
@@ -332,7 +341,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
       if (q == k)
       {
         const int off = a.off1d[rr[q] * M + c];
-        if (q < 2)
+        if (q < K0)
           v = off;
         else
           v = (off < 0) ? 0 : (off - cbk[q]) * T * nnp + fbk[q];
@@ -340,7 +349,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     loc[e] = v;
   }
 #pragma unroll
-  for (int k = 2; k < 6; k++)
+  for (int k = K0; k < 6; k++)
   {
     const int len = (a.conn_pntr[rr[k] + 1] - cbk[k]) * T * nnp;
     const double *src = a.vAk + (int64_t(k) * a.nconn + cbk[k]) * tstride;
@@ -382,9 +391,10 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     return vAk + (int64_t(k) * a.nconn + conn) * tstride + int64_t(t) * nnp;
   };
   // the group's dims 2, 3 entries are read from LDS when it opens and used
-  // when it closes; lanes of terms past T read term 0 (their sums are zero)
+  // when it closes; lanes of terms past T read term 0
   auto open_23 = [&](int c2, int c3) {
-    const int l2 = loc[2 * M + c2], l3 = loc[3 * M + c3];
+    const int l2 = __builtin_amdgcn_readfirstlane(loc[2 * M + c2]);
+    const int l3 = __builtin_amdgcn_readfirstlane(loc[3 * M + c3]);
 #pragma unroll
     for (int b = 0; b < TG; b++)
     {
@@ -393,8 +403,8 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     }
   };
   auto open_01 = [&](int c0, int c1) {
-    q0 = loc[c0];
-    q1 = loc[M + c1];
+    q0 = __builtin_amdgcn_readfirstlane(loc[c0]);
+    q1 = __builtin_amdgcn_readfirstlane(loc[M + c1]);
   };
   // the open (c0..c3) group: its stage-1 sums through K23 into acc2
   auto close_23 = [&]() {
@@ -415,10 +425,11 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
 #pragma unroll
     for (int b = 0; b < TG; b++)
     {
-      // dims 0, 1 blocks, read from global memory once per (c0,c1) group
+      // dims 0, 1 blocks, once per (c0,c1) group: from LDS when staged
+      // there, otherwise from global memory
       double k0r[4], k1r[4];
-      const double *p0 = block_of(0, q0, tcl[b]);
-      const double *p1 = block_of(1, q1, tcl[b]);
+      const double *p0 = ALL6 ? fac + q0 + tcl[b] * nnp : block_of(0, q0, tcl[b]);
+      const double *p1 = ALL6 ? fac + q1 + tcl[b] * nnp : block_of(1, q1, tcl[b]);
 #pragma unroll
       for (int q = 0; q < 4; q++)
       {
@@ -478,54 +489,60 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
       }
     };
     fetch_factors(0);
-    for (int jb = 0; jb < nb; jb += XD)
+    // one column: group changes, then its stage-1 product; the next
+    // column's dimension-4/5 entries go out before the group work
+    auto column = [&](int jj, double xv) {
+      double bv[TG];
+#pragma unroll
+      for (int b = 0; b < TG; b++)
+        bv[b] = f4n[b] * f5n[b];
+      fetch_factors((jj + 1 < nb) ? jj + 1 : jj);
+      const unsigned long long w =
+          (static_cast<unsigned long long>(
+               static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
+           << 32) |
+          static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
+      const unsigned long long p03 = w & mask03;
+      if (p03 != prev03)
+      {
+        if (prev03 != none)
+          close_23();
+        const unsigned long long p01 = w & mask01;
+        if (p01 != prev01)
+        {
+          if (prev01 != none)
+            close_01();
+          open_01(packed_1d(w, 0), packed_1d(w, 1));
+          prev01 = p01;
+        }
+        open_23(packed_1d(w, 2), packed_1d(w, 3));
+        prev03 = p03;
+      }
+#pragma unroll
+      for (int b = 0; b < TG; b++)
+        d1[b] = mfma16(xv, bv[b], d1[b]);
+    };
+    // full chunks of XD columns: every slot of the x ring is consumed and
+    // refilled once per chunk, unconditionally (the batch's last chunk
+    // re-reads its last column), so no branch joins an x load in flight
+    int jb = 0;
+    for (; jb + XD <= nb; jb += XD)
     {
 #pragma unroll
       for (int u = 0; u < XD; u++)
       {
-        const int jj = jb + u;
-        if (jj < nb)
-        {
-          const double xv = xq[u];
-          double bv[TG];
-#pragma unroll
-          for (int b = 0; b < TG; b++)
-            bv[b] = tok[b] ? f4n[b] * f5n[b] : 0.0;
-          fetch_factors((jj + 1 < nb) ? jj + 1 : jj);
-          {
-            // refill unconditionally (the tail re-reads the batch's last
-            // column) so that no branch joins an x load in flight
-            const int nxt = (jj + XD < nb) ? jj + XD : nb - 1;
-            const int col = __builtin_amdgcn_readlane(colv, nxt);
-            xq[u] = a.x[int64_t(col) * N + xoff];
-          }
-          const unsigned long long w =
-              (static_cast<unsigned long long>(
-                   static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
-               << 32) |
-              static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
-          const unsigned long long p03 = w & mask03;
-          if (p03 != prev03)
-          {
-            if (prev03 != none)
-              close_23();
-            const unsigned long long p01 = w & mask01;
-            if (p01 != prev01)
-            {
-              if (prev01 != none)
-                close_01();
-              open_01(packed_1d(w, 0), packed_1d(w, 1));
-              prev01 = p01;
-            }
-            open_23(packed_1d(w, 2), packed_1d(w, 3));
-            prev03 = p03;
-          }
-#pragma unroll
-          for (int b = 0; b < TG; b++)
-            d1[b] = mfma16(xv, bv[b], d1[b]);
-        }
+        const double xv = xq[u];
+        const int nxt = (jb + u + XD < nb) ? jb + u + XD : nb - 1;
+        const int col = __builtin_amdgcn_readlane(colv, nxt);
+        xq[u] = a.x[int64_t(col) * N + xoff];
+        column(jb + u, xv);
       }
     }
+    // the last nb - jb (< XD) columns, already in the ring
+#pragma unroll
+    for (int u = 0; u < XD; u++)
+      if (jb + u < nb)
+        column(jb + u, xq[u]);
   }
   if (prev03 != none)
   {
@@ -534,15 +551,17 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
   }
 
   // terms: blocks in order, then the four t' of a lane row by a fixed tree;
-  // lanes with t' = 0 hold entry (j0,j1, j2,j3 = hi, j4,j5 = lane & 3)
+  // lanes with t' = 0 hold entry (j0,j1, j2,j3 = hi, j4,j5 = lane & 3).
+  // Lanes of terms past T carried term 0's factors through every stage
+  // (stage results never cross lanes of different t'); they are dropped here.
   double v[4];
 #pragma unroll
   for (int q = 0; q < 4; q++)
   {
-    double s = acc3[0][q];
+    double s = tok[0] ? acc3[0][q] : 0.0;
 #pragma unroll
     for (int b = 1; b < TG; b++)
-      s += acc3[b][q];
+      s += tok[b] ? acc3[b][q] : 0.0;
     s += __shfl_xor(s, 4, wave_size);
     s += __shfl_xor(s, 8, wave_size);
     v[q] = s;
@@ -564,11 +583,12 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
   }
 }
 
-template<int TG>
-bool launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
+template<int TG, bool ALL6>
+bool launch_mfma6_tables(kron_args<double> const &a, int num_rows, int waves,
+                         hipStream_t stream)
 {
   auto go = [&](auto kern, int W) -> bool {
-    const size_t lds = mfma_d6n2_lds_bytes(a, W);
+    const size_t lds = mfma_d6n2_lds_bytes(a, W, ALL6);
     if (lds > 160 * 1024)
       return false;
     if (lds > 64 * 1024 &&
@@ -585,12 +605,23 @@ bool launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hip
   switch (waves)
   {
   // XD x loads in flight per wavefront; 16 wavefronts keep 4 to fit 128 VGPRs
-  case 1: return go(kron_mfma_d6n2<1, TG, 8>, 1);
-  case 2: return go(kron_mfma_d6n2<2, TG, 8>, 2);
-  case 8: return go(kron_mfma_d6n2<8, TG, 8>, 8);
-  case 16: return go(kron_mfma_d6n2<16, TG, 4>, 16);
-  default: return go(kron_mfma_d6n2<4, TG, 8>, 4);
+  case 1: return go(kron_mfma_d6n2<1, TG, 8, ALL6>, 1);
+  case 2: return go(kron_mfma_d6n2<2, TG, 16, ALL6>, 2);
+  case 8: return go(kron_mfma_d6n2<8, TG, D6_XD8, ALL6>, 8);
+  case 16: return go(kron_mfma_d6n2<16, TG, 4, ALL6>, 16);
+  default: return go(kron_mfma_d6n2<4, TG, 16, ALL6>, 4);
   }
+}
+
+//! All six dimensions' tables in LDS when they fit (no global loads but the
+//! x slices), else dimensions 2..5.
+template<int TG>
+bool launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hipStream_t stream)
+{
+  const int W = (waves == 1 || waves == 2 || waves == 8 || waves == 16) ? waves : 4;
+  if (mfma_d6n2_lds_bytes(a, W, true) <= 160 * 1024)
+    return launch_mfma6_tables<TG, true>(a, num_rows, waves, stream);
+  return launch_mfma6_tables<TG, false>(a, num_rows, waves, stream);
 }
 
 inline bool launch_mfma_d6n2(kron_args<double> const &a, int num_rows, int waves,
@@ -626,7 +657,8 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d4n2(kron_args<double>
   constexpr int N = 16;
   __shared__ double red[W * wave_size];
   const int lane = threadIdx.x & (wave_size - 1);
-  const int wave = threadIdx.x / wave_size;
+  // uniform per wavefront: loop bounds and group keys stay scalar
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / wave_size);
   const int q = lane >> 4, g = (lane >> 2) & 3, m = lane & 3;
   const int T = a.T;
 
