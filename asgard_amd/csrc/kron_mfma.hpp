@@ -392,9 +392,9 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
   };
   // the group's dims 2, 3 entries are read from LDS when it opens and used
   // when it closes; lanes of terms past T read term 0
-  auto open_23 = [&](int c2, int c3) {
-    const int l2 = __builtin_amdgcn_readfirstlane(loc[2 * M + c2]);
-    const int l3 = __builtin_amdgcn_readfirstlane(loc[3 * M + c3]);
+  // l23: the (c0..c3) group's LDS offsets of dims 2 (low 16 bits) and 3
+  auto open_23 = [&](unsigned int l23) {
+    const int l2 = static_cast<int>(l23 & 0xFFFFu), l3 = static_cast<int>(l23 >> 16);
 #pragma unroll
     for (int b = 0; b < TG; b++)
     {
@@ -402,9 +402,10 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
       a3v[b] = fac[l3 + tcl[b] * nnp + elo];
     }
   };
-  auto open_01 = [&](int c0, int c1) {
-    q0 = __builtin_amdgcn_readfirstlane(loc[c0]);
-    q1 = __builtin_amdgcn_readfirstlane(loc[M + c1]);
+  // l01: the (c0,c1) group's dims 0 (low 16 bits) and 1 entries of loc
+  auto open_01 = [&](unsigned int l01) {
+    q0 = static_cast<int>(l01 & 0xFFFFu);
+    q1 = static_cast<int>(l01 >> 16);
   };
   // the open (c0..c3) group: its stage-1 sums through K23 into acc2
   auto close_23 = [&]() {
@@ -455,17 +456,22 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
   unsigned long long prev01 = none, prev03 = none;
   for (int64_t j0 = cb; j0 < ce; j0 += wave_size)
   {
-    // one batch of up to 64 columns, one per lane: id, packed 1D indices,
-    // LDS offsets of the dimension-4 and -5 blocks
+    // one batch of up to 64 columns, one per lane: id, packed 1D indices
+    // (the group key) and the loc entries of its six 1D indices, two per word
     const int64_t jl = j0 + lane;
-    int colv = 0, l4v = 0, l5v = 0;
-    unsigned long long wv = 0;
+    int colv = 0;
+    unsigned int l01v = 0, l23v = 0, l45v = 0;
+    unsigned long long wv = none;
     if (jl < ce)
     {
       colv = a.indx[jl];
       wv = a.epack[colv];
-      l4v = loc[4 * M + packed_1d(wv, 4)];
-      l5v = loc[5 * M + packed_1d(wv, 5)];
+      l01v = static_cast<unsigned int>(loc[packed_1d(wv, 0)]) |
+             (static_cast<unsigned int>(loc[M + packed_1d(wv, 1)]) << 16);
+      l23v = static_cast<unsigned int>(loc[2 * M + packed_1d(wv, 2)]) |
+             (static_cast<unsigned int>(loc[3 * M + packed_1d(wv, 3)]) << 16);
+      l45v = static_cast<unsigned int>(loc[4 * M + packed_1d(wv, 4)]) |
+             (static_cast<unsigned int>(loc[5 * M + packed_1d(wv, 5)]) << 16);
     }
     const unsigned int wlo = static_cast<unsigned int>(wv), whi = static_cast<unsigned int>(wv >> 32);
     const int nb = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
@@ -479,8 +485,8 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     // dimension-4/5 entries of the next column, read one column ahead
     double f4n[TG], f5n[TG];
     auto fetch_factors = [&](int jj) {
-      const int l4 = __builtin_amdgcn_readlane(l4v, jj);
-      const int l5 = __builtin_amdgcn_readlane(l5v, jj);
+      const unsigned int l45 = static_cast<unsigned int>(__builtin_amdgcn_readlane(l45v, jj));
+      const int l4 = static_cast<int>(l45 & 0xFFFFu), l5 = static_cast<int>(l45 >> 16);
 #pragma unroll
       for (int b = 0; b < TG; b++)
       {
@@ -512,10 +518,10 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
         {
           if (prev01 != none)
             close_01();
-          open_01(packed_1d(w, 0), packed_1d(w, 1));
+          open_01(static_cast<unsigned int>(__builtin_amdgcn_readlane(l01v, jj)));
           prev01 = p01;
         }
-        open_23(packed_1d(w, 2), packed_1d(w, 3));
+        open_23(static_cast<unsigned int>(__builtin_amdgcn_readlane(l23v, jj)));
         prev03 = p03;
       }
 #pragma unroll
@@ -621,6 +627,9 @@ bool launch_mfma6_terms(kron_args<double> const &a, int num_rows, int waves, hip
   const int W = (waves == 1 || waves == 2 || waves == 8 || waves == 16) ? waves : 4;
   if (mfma_d6n2_lds_bytes(a, W, true) <= 160 * 1024)
     return launch_mfma6_tables<TG, true>(a, num_rows, waves, stream);
+  // dimensions 0, 1 then go by connect_1d position, packed in 16 bits
+  if (a.nconn > 0xFFFF)
+    return false;
   return launch_mfma6_tables<TG, false>(a, num_rows, waves, stream);
 }
 
