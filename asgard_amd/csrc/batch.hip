@@ -31,6 +31,9 @@
 
 namespace akm
 {
+//! Arnoldi-iteration graph replays between two host reads of the batch state.
+constexpr int bt_chunk = 8;
+
 constexpr int bt_threads = 256;
 constexpr int bt_cols = 8;
 constexpr int64_t bt_rows_per_block = 2048;
@@ -756,9 +759,15 @@ akm_status akm_batch_solve(akm_batch b, double *const *x, const double *const *r
     }
     if (cycles >= max_cycles)
       break;
-    for (int j = 0; j < b->restart; j++)
+    // Arnoldi iterations in chunks of bt_chunk graph replays between host
+    // reads of the per-system flags: a system that converges inside a chunk
+    // has cleared its own flag, and the kernels skip it from then on
+    for (int j = 0; j < b->restart;)
     {
-      AKM_HIP(hipGraphLaunch(b->g_iter, s));
+      const int k = std::min(bt_chunk, b->restart - j);
+      for (int q = 0; q < k; q++)
+        AKM_HIP(hipGraphLaunch(b->g_iter, s));
+      j += k;
       AKM_TRY(b->read_state());
       int still = 0;
       for (int q = 0; q < b->nsys; q++)

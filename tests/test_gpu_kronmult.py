@@ -140,7 +140,8 @@ def test_autotune_keeps_parity(d, n, L, T):
     x = dev(c["x"])
     scratch = dev(np.zeros_like(c["x"]))
     waves, lds = m.autotune(x, scratch)
-    assert waves in (1, 2, 4, 8)
+    # 16 wavefronts per row exist for the matrix-core kernels (kernel 2) only
+    assert waves in (1, 2, 4, 8) or (waves == 16 and lds == 2)
     # a second operator of the same shape reuses the cached choice
     m2 = asgard_amd.KronmultMatrix.sparse_grid(d, n, c["table"], L, T, c["vA"])
     assert m2.autotune(x, scratch) == (waves, lds)
