@@ -390,16 +390,22 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
   auto block_of = [&](int k, int conn, int t) -> const double * {
     return vAk + (int64_t(k) * a.nconn + conn) * tstride + int64_t(t) * nnp;
   };
+  // LDS factor reads of lane term t = 4 b + t': lanes of terms past T read
+  // past their connection's T blocks, into the next connection's blocks or,
+  // at the end of the tables, the row-sum area (at most 12 values, always
+  // inside the workgroup's LDS); whatever they read stays in their own
+  // lanes through every stage and is dropped at the end
+  constexpr int bstride = 4 * nnp; // LDS distance between the blocks b, b + 1
   // the group's dims 2, 3 entries are read from LDS when it opens and used
-  // when it closes; lanes of terms past T read term 0
-  // l23: the (c0..c3) group's LDS offsets of dims 2 (low 16 bits) and 3
+  // when it closes; l23: the group's LDS offsets of dims 2 (low 16 bits) and 3
   auto open_23 = [&](unsigned int l23) {
-    const int l2 = static_cast<int>(l23 & 0xFFFFu), l3 = static_cast<int>(l23 >> 16);
+    const double *p2 = fac + static_cast<int>(l23 & 0xFFFFu) + tq * nnp + ehi;
+    const double *p3 = fac + static_cast<int>(l23 >> 16) + tq * nnp + elo;
 #pragma unroll
     for (int b = 0; b < TG; b++)
     {
-      a2v[b] = fac[l2 + tcl[b] * nnp + ehi];
-      a3v[b] = fac[l3 + tcl[b] * nnp + elo];
+      a2v[b] = p2[b * bstride];
+      a3v[b] = p3[b * bstride];
     }
   };
   // l01: the (c0,c1) group's dims 0 (low 16 bits) and 1 entries of loc
@@ -429,8 +435,9 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
       // dims 0, 1 blocks, once per (c0,c1) group: from LDS when staged
       // there, otherwise from global memory
       double k0r[4], k1r[4];
-      const double *p0 = ALL6 ? fac + q0 + tcl[b] * nnp : block_of(0, q0, tcl[b]);
-      const double *p1 = ALL6 ? fac + q1 + tcl[b] * nnp : block_of(1, q1, tcl[b]);
+      // (global reads keep lanes past T on term 0, inside the array)
+      const double *p0 = ALL6 ? fac + q0 + tq * nnp + b * bstride : block_of(0, q0, tcl[b]);
+      const double *p1 = ALL6 ? fac + q1 + tq * nnp + b * bstride : block_of(1, q1, tcl[b]);
 #pragma unroll
       for (int q = 0; q < 4; q++)
       {
@@ -486,12 +493,13 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     double f4n[TG], f5n[TG];
     auto fetch_factors = [&](int jj) {
       const unsigned int l45 = static_cast<unsigned int>(__builtin_amdgcn_readlane(l45v, jj));
-      const int l4 = static_cast<int>(l45 & 0xFFFFu), l5 = static_cast<int>(l45 >> 16);
+      const double *p4 = fac + static_cast<int>(l45 & 0xFFFFu) + tq * nnp + ehi;
+      const double *p5 = fac + static_cast<int>(l45 >> 16) + tq * nnp + elo;
 #pragma unroll
       for (int b = 0; b < TG; b++)
       {
-        f4n[b] = fac[l4 + tcl[b] * nnp + ehi];
-        f5n[b] = fac[l5 + tcl[b] * nnp + elo];
+        f4n[b] = p4[b * bstride];
+        f5n[b] = p5[b * bstride];
       }
     };
     fetch_factors(0);

@@ -451,6 +451,34 @@ def test_mfma_d6n2_vlasov_sampled_rows():
     assert rel(got, ref) < TOL64
 
 
+@pytest.mark.parametrize("waves", [8, 16])
+def test_mfma_d6n2_eight_terms_tables_past_lds(waves):
+    """d=6 L=7 with 8 random terms: all six dimensions' factor tables of a
+    coarse row need about 196 KB, more than a CU's LDS, so the kernel stages
+    dimensions 2-5 only and reads dimensions 0, 1 from global memory; checked
+    on sampled rows of every length against the oracle."""
+    from asgard_amd import grid
+
+    d, n, L, T = 6, 2, 7, 8
+    table, _ = grid.element_table([L] * d)
+    rng = np.random.default_rng(21)
+    s = n << L
+    ops = [np.asfortranarray(rng.uniform(-1, 1, (s, s))) for _ in range(T * d)]
+    vA = grid.build_vA(d, n, T, L, ops)
+    m = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
+    m.set_kernel_family(1)
+    m.set_tuning(waves_per_row=waves)
+    x = rng.uniform(-1, 1, table.shape[0] * n ** d)
+    y = dev(np.zeros_like(x))
+    m.apply(1.0, dev(x), 0.0, y)
+    rows = np.linspace(0, table.shape[0] - 1, 8).astype(np.int32)
+    pntr, indx, iA = oracle.sparse_pattern_rows(table, rows, n=n, T=T, max_level=L)
+    ref = np.zeros(len(rows) * n ** d)
+    oracle.kronmult_sparse(d, n, pntr, indx, T, iA, vA, 1.0, x, 0.0, ref)
+    got = host(y).reshape(-1, n ** d)[rows].reshape(-1)
+    assert rel(got, ref) < TOL64
+
+
 def test_mfma_kernel_family_rejected_elsewhere():
     c = sparse_grid_case(5, 2, 3, 1, seed=7)
     m = asgard_amd.KronmultMatrix.sparse_grid(5, 2, c["table"], 3, 1, c["vA"])
