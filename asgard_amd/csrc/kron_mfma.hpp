@@ -460,11 +460,14 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
   };
 
   constexpr unsigned long long mask01 = (1ull << 20) - 1, mask03 = (1ull << 40) - 1, none = ~0ull;
-  unsigned long long prev01 = none, prev03 = none;
+  // key of the column before the batch (none before the wavefront's first)
+  unsigned int last_lo = 0xFFFFFFFFu, last_hi = 0xFFFFFFFFu;
+  int opened = 0; // a (c0..c3) group is open
   for (int64_t j0 = cb; j0 < ce; j0 += wave_size)
   {
-    // one batch of up to 64 columns, one per lane: id, packed 1D indices
-    // (the group key) and the loc entries of its six 1D indices, two per word
+    // one batch of up to 64 columns, one per lane: id, the loc entries of
+    // its six 1D indices (two per word), and in bits 30 and 31 of the
+    // dims-4/5 word whether it opens a new (c0..c3) / (c0,c1) group
     const int64_t jl = j0 + lane;
     int colv = 0;
     unsigned int l01v = 0, l23v = 0, l45v = 0;
@@ -478,9 +481,24 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
       l23v = static_cast<unsigned int>(loc[2 * M + packed_1d(wv, 2)]) |
              (static_cast<unsigned int>(loc[3 * M + packed_1d(wv, 3)]) << 16);
       l45v = static_cast<unsigned int>(loc[4 * M + packed_1d(wv, 4)]) |
-             (static_cast<unsigned int>(loc[5 * M + packed_1d(wv, 5)]) << 16);
+             (static_cast<unsigned int>(loc[5 * M + packed_1d(wv, 5)]) << 15);
     }
-    const unsigned int wlo = static_cast<unsigned int>(wv), whi = static_cast<unsigned int>(wv >> 32);
+    {
+      const unsigned int wlo = static_cast<unsigned int>(wv), whi = static_cast<unsigned int>(wv >> 32);
+      unsigned int plo = __shfl_up(wlo, 1, wave_size), phi = __shfl_up(whi, 1, wave_size);
+      if (lane == 0)
+      {
+        plo = last_lo;
+        phi = last_hi;
+      }
+      const unsigned long long pv = (static_cast<unsigned long long>(phi) << 32) | plo;
+      const bool first = (pv == none);
+      l45v |= ((first || ((wv ^ pv) & mask03) != 0) ? (1u << 30) : 0u) |
+              ((first || ((wv ^ pv) & mask01) != 0) ? (1u << 31) : 0u);
+      const int nb0 = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
+      last_lo = static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, nb0 - 1));
+      last_hi = static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, nb0 - 1));
+    }
     const int nb = static_cast<int>((ce - j0 < wave_size) ? ce - j0 : wave_size);
     double xq[XD];
 #pragma unroll
@@ -491,10 +509,13 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
     }
     // dimension-4/5 entries of the next column, read one column ahead
     double f4n[TG], f5n[TG];
+    // the dims-4/5 word of the next column (its factors are read one column
+    // ahead; its group bits are used when it is processed)
+    unsigned int l45n = 0;
     auto fetch_factors = [&](int jj) {
-      const unsigned int l45 = static_cast<unsigned int>(__builtin_amdgcn_readlane(l45v, jj));
-      const double *p4 = fac + static_cast<int>(l45 & 0xFFFFu) + tq * nnp + ehi;
-      const double *p5 = fac + static_cast<int>(l45 >> 16) + tq * nnp + elo;
+      l45n = static_cast<unsigned int>(__builtin_amdgcn_readlane(l45v, jj));
+      const double *p4 = fac + static_cast<int>(l45n & 0x7FFFu) + tq * nnp + ehi;
+      const double *p5 = fac + static_cast<int>((l45n >> 15) & 0x7FFFu) + tq * nnp + elo;
 #pragma unroll
       for (int b = 0; b < TG; b++)
       {
@@ -510,27 +531,20 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
 #pragma unroll
       for (int b = 0; b < TG; b++)
         bv[b] = f4n[b] * f5n[b];
+      const unsigned int flags = l45n >> 30; // this column's group bits
       fetch_factors((jj + 1 < nb) ? jj + 1 : jj);
-      const unsigned long long w =
-          (static_cast<unsigned long long>(
-               static_cast<unsigned int>(__builtin_amdgcn_readlane(whi, jj)))
-           << 32) |
-          static_cast<unsigned int>(__builtin_amdgcn_readlane(wlo, jj));
-      const unsigned long long p03 = w & mask03;
-      if (p03 != prev03)
+      if (flags & 1u)
       {
-        if (prev03 != none)
+        if (opened)
           close_23();
-        const unsigned long long p01 = w & mask01;
-        if (p01 != prev01)
+        if (flags & 2u)
         {
-          if (prev01 != none)
+          if (opened)
             close_01();
           open_01(static_cast<unsigned int>(__builtin_amdgcn_readlane(l01v, jj)));
-          prev01 = p01;
         }
         open_23(static_cast<unsigned int>(__builtin_amdgcn_readlane(l23v, jj)));
-        prev03 = p03;
+        opened = 1;
       }
 #pragma unroll
       for (int b = 0; b < TG; b++)
@@ -558,7 +572,7 @@ __global__ void __launch_bounds__(W *wave_size) kron_mfma_d6n2(kron_args<double>
       if (jb + u < nb)
         column(jb + u, xq[u]);
   }
-  if (prev03 != none)
+  if (opened)
   {
     close_23();
     close_01();
