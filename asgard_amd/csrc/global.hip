@@ -455,24 +455,370 @@ __global__ void __launch_bounds__(gk_threads)
   }
 }
 
-//! y = (alpha * sum over chains of the chain's last stage (x for a term of
-//! identity factors), in chain order, + beta*y + gamma*xg) / (1 - jdt * pc)
-//! (no division when pc is null).
+// ---------------------------------------------------------------------------
+// Slab layout.  The terms with the same one or two non-identity dimensions S
+// form a group.  A slab of the group is a class of cells equal in every 1D
+// index outside S; every stage of the group's chains maps a slab into
+// itself (a stage along k in S reads only the cell's fibre along k), so one
+// workgroup loads a slab's x once, runs all the group's chains on it in LDS
+// and writes the group's sum once.  Small slabs are bundled into one
+// workgroup; a slab too large for LDS is split over the other-index
+// combinations (the tensor entries outside S, on which the group's factors
+// act as the identity).  Per pass the 1D factor blocks of one (term,
+// dimension), restricted to the 1D indices below the bundle's level bound,
+// and the bundle's connection lists along that dimension are staged in LDS
+// as well, so the connection loops touch only LDS.
+
+// a bundle holds at most 256 blocks (one per thread), a large slab alone up
+// to 1024
+constexpr int gs_blocks_small = 256;
+constexpr int gs_blocks_large = 1024;
+constexpr int gs_lds_max = 160 * 1024;
+constexpr int gs_split_level = 5; // slabs with level bound >= this run one chain per workgroup
+constexpr int gs_side_level = 6;  // split items of slabs from this level bound on: side stream
+constexpr int gs_lds_resident = 64 * 1024; // stage everything at once up to this size
+constexpr int gs_unroll = 8;               // independent loads per thread while staging
+
+//! Consecutive slabs of one group, their cells in bundle-local order.
+struct gs_bundle
+{
+  int group;
+  int cell0;    // first entry in gs_args::cells
+  int ncell;
+  int C;        // other-index combinations per chunk
+  int Lp;       // level bound of the slab dimensions (compact factor table)
+  int resident; // 1: every factor table and both lists staged once
+  int list0;    // first int of the bundle's list images in gs_args::lists
+  int llen[2];  // per slab dimension: image length (starts, diagonals, entries)
+};
+
+//! A group: its slab dimensions, factor tables and passes.
+struct gs_group
+{
+  int nd;           // slab dimensions (1 or 2)
+  int dim[2];       // ascending
+  int str[2];       // their strides in the cell tensor
+  int call;         // other-index combinations, n^(d - nd)
+  int off0;         // first of its call entries in gs_args::offs
+  int tab0, ntab;   // its (term, dimension) factor tables in gs_args::tabs
+  int pass0, npass; // its passes in gs_args::passes
+  int slot;         // work-buffer slot of the group's sum
+};
+
+//! One pass: a chain's stage.  mode 0: x -> sum (one-stage chain); 1: x ->
+//! intermediate; 2: intermediate -> sum.
+struct gs_pass
+{
+  int tab;  // factor table index within the group
+  int j;    // slab dimension index (0 or 1)
+  int fill, mode;
+};
+
+//! One workgroup's work: a chunk of a bundle with its group's data, so that
+//! one record read starts it.
+struct gs_item
+{
+  int cell0, ncell; // the bundle's cells in gs_args::cells
+  int C;            // other-index combinations per chunk
+  int cb, nq;       // the group's first compact factor table in gs_ctab, nq blocks each
+  int resident;     // 1: every factor table and both lists staged once
+  int list0;        // first int of the bundle's list images
+  int llen[2];      // per slab dimension: image length
+  int nd;           // slab dimensions (1 or 2)
+  int str[2];       // their strides in the cell tensor
+  int off;          // first of the chunk's C entries in gs_args::offs
+  int tab0, ntab;   // the group's factor tables in gs_args::tabs
+  int pass0, npass; // the group's passes in gs_args::passes
+  int slot;         // work-buffer slot of the group's sum
+};
+
+struct gs_args
+{
+  int d, n, N, T;
+  int64_t active_cells;
+  int64_t stride; // work-buffer slot size
+  const gs_pass *passes;
+  const int *cells; // bundle cells (padded cell indices)
+  // per bundle and slab dimension, the LDS image of its lists: ncell + 1
+  // starts and ncell diagonal positions (relative to the entries), then
+  // the entries (the connected cell's element offset in the bundle's x |
+  // its factor block's element offset in the compact table << 16)
+  const int *lists;
+  const int *offs; // tensor offsets of the other-index combinations
+};
+
+//! LDS bytes of one work item: x (and the intermediate when nd = 2), ntab
+//! factor tables of nq blocks, then the int lists (nlist_ints).
+__host__ __device__ inline int64_t gs_lds_bytes(int esize, int nd, int n, int ncell, int C, int nq,
+                                                int ntab, int64_t nlist_ints, int npass)
+{
+  const int64_t ds = (nd == 1) ? n : int64_t(n) * n;
+  const int64_t bd = int64_t(ncell) * ds * C;
+  const int64_t vals = bd * nd + int64_t(ntab) * nq * n * n;
+  return int64_t(16) * npass + (vals * esize + 15) / 16 * 16 + 4 * nlist_ints;
+}
+
+//! dst[i] = src[i], i < count, by a workgroup of TPB threads with
+//! gs_unroll loads per thread in flight.
+template<int TPB, typename T>
+__device__ __forceinline__ void gs_copy(T *dst, const T *__restrict__ src, int count)
+{
+  for (int base = threadIdx.x; base < count; base += gs_unroll * TPB)
+  {
+    T v[gs_unroll];
+#pragma unroll
+    for (int u = 0; u < gs_unroll; u++)
+    {
+      const int i = base + u * TPB;
+      v[u] = (i < count) ? src[i] : T{};
+    }
+#pragma unroll
+    for (int u = 0; u < gs_unroll; u++)
+    {
+      const int i = base + u * TPB;
+      if (i < count)
+        dst[i] = v[u];
+    }
+  }
+}
+
+//! One connection's block product along slab dimension J of ND: v[s] +=
+//! sum_r A(s_J, r) * xb[s with s_J = r] over r in [rlo(s_J), rhi(s_J)): all
+//! columns (FILL_FULL), from the row's own index on (upper diagonal block),
+//! or before it (lower diagonal block).  Blocks are n^ND values, the last
+//! slab digit fastest; A(s, r) = A[s + r * NT].
+enum gs_cols : int
+{
+  gs_all = 0,
+  gs_from_diag = 1,
+  gs_before_diag = 2,
+};
+
+template<typename P, int NT, int ND, int J, int COLS>
+__device__ __forceinline__ void gs_block_fma(const P (&av)[NT * NT],
+                                             const P (&xv)[(ND == 1) ? NT : NT * NT], P *v)
+{
+  constexpr int ds = (ND == 1) ? NT : NT * NT;
+#pragma unroll
+  for (int si = 0; si < ds; si++)
+  {
+    const int s0 = (ND == 1) ? si : si / NT;
+    const int s1 = (ND == 1) ? 0 : si % NT;
+    const int sj = (J == 0) ? s0 : s1;
+#pragma unroll
+    for (int r = 0; r < NT; r++)
+    {
+      if ((COLS == gs_from_diag && r < sj) || (COLS == gs_before_diag && r >= sj))
+        continue;
+      const int xi = (ND == 1) ? r : ((J == 0) ? r * NT + s1 : s0 * NT + r);
+      v[si] = fma(av[sj + r * NT], xv[xi], v[si]);
+    }
+  }
+}
+
+//! The factor block and the x block of one list entry.
+template<typename P, int NT, int ND>
+__device__ __forceinline__ void gs_block_load(const P *tab, const P *in, unsigned e, int ob,
+                                              P (&av)[NT * NT],
+                                              P (&xv)[(ND == 1) ? NT : NT * NT])
+{
+  constexpr int ds = (ND == 1) ? NT : NT * NT;
+  const P *A = tab + (e >> 16);
+  const P *xb = in + (e & 0xffffu) + ob;
+#pragma unroll
+  for (int k = 0; k < NT * NT; k++)
+    av[k] = A[k];
+#pragma unroll
+  for (int k = 0; k < ds; k++)
+    xv[k] = xb[k];
+}
+
+//! One pass along slab dimension J for one block (cell c >= 0, block base
+//! ob within the cell): the fill part of the staged factor applied to `in`.
+//! List entries hold the connected cell's element offset in `in` (low 16
+//! bits) and its factor block's offset in `tab` (high 16 bits).  Each
+//! value's sum runs over the connections ascending, then the factor's
+//! columns, as in gk_cell_part; the diagonal block, restricted by the fill,
+//! is the first connection of an upper fill and the last of a lower one.
+//! The loop is software-pipelined: the blocks of the next connection and
+//! the list entry after it are read while this connection's products run.
+template<typename P, int NT, int ND, int J>
+__device__ __forceinline__ void gs_pass_block(const P *in, const P *tab, const int *st,
+                                              const int *dg, const unsigned *ent, int c, int ob,
+                                              int fill, P *v)
+{
+  constexpr int ds = (ND == 1) ? NT : NT * NT;
+  const int d0 = dg[c];
+  const int lo = (fill == gk_upper) ? d0 + 1 : st[c];
+  const int hi = (fill == gk_lower) ? d0 : st[c + 1];
+  P av[NT * NT], xv[ds];
+  if (fill == gk_upper)
+  {
+    gs_block_load<P, NT, ND>(tab, in, ent[d0], ob, av, xv);
+    gs_block_fma<P, NT, ND, J, gs_from_diag>(av, xv, v);
+  }
+  if (lo < hi)
+  {
+    gs_block_load<P, NT, ND>(tab, in, ent[lo], ob, av, xv);
+    unsigned en = ent[(lo + 1 < hi) ? lo + 1 : d0];
+    for (int p = lo; p < hi; p++)
+    {
+      P an[NT * NT], xn[ds];
+      gs_block_load<P, NT, ND>(tab, in, en, ob, an, xn);
+      en = ent[(p + 2 < hi) ? p + 2 : d0];
+      gs_block_fma<P, NT, ND, J, gs_all>(av, xv, v);
+#pragma unroll
+      for (int k = 0; k < NT * NT; k++)
+        av[k] = an[k];
+#pragma unroll
+      for (int k = 0; k < ds; k++)
+        xv[k] = xn[k];
+    }
+  }
+  if (fill == gk_lower)
+  {
+    gs_block_load<P, NT, ND>(tab, in, ent[d0], ob, av, xv);
+    gs_block_fma<P, NT, ND, J, gs_before_diag>(av, xv, v);
+  }
+}
+
+//! One work item (bundle, chunk) per workgroup: every chain of the bundle's
+//! group on the bundle's slabs, the group's sum written to its slot (active
+//! cells only).  Chunk entries are blocks of n^ND values (the slab digits,
+//! the last fastest), one per other-index combination of the chunk; a
+//! thread owns one block.  A resident item stages x, every factor table of
+//! the group and both dimensions' lists once; otherwise a pass stages its
+//! own table and list when they change.
+template<typename P, int NT, int ND, int TPB>
+__global__ void __launch_bounds__(TPB)
+    gk_slab(gs_args a, const P *__restrict__ ctab, const gs_item *__restrict__ items,
+            const P *__restrict__ x, P *work)
+{
+  extern __shared__ __align__(16) unsigned char gs_smem[];
+  constexpr int nn = NT * NT;
+  constexpr int ds = (ND == 1) ? NT : nn;
+  const gs_item b = items[blockIdx.x];
+  const int C = b.C;
+  const int D = ds * C;
+  const int BD = b.ncell * D;
+  const int nq = b.nq;
+  const int ntab_lds = b.resident ? b.ntab : 1;
+  gs_pass *pass = reinterpret_cast<gs_pass *>(gs_smem);
+  P *X = reinterpret_cast<P *>(gs_smem + 16 * b.npass);
+  P *Z = X + BD; // used when ND = 2
+  P *tab = X + int64_t(BD) * ND;
+  int *lists = reinterpret_cast<int *>(
+      gs_smem + 16 * b.npass +
+      ((int64_t(BD) * ND + int64_t(ntab_lds) * nq * nn) * int64_t(sizeof(P)) + 15) / 16 * 16);
+  gs_copy<TPB>(reinterpret_cast<int *>(pass), reinterpret_cast<const int *>(a.passes + b.pass0),
+               4 * b.npass);
+
+  // this thread's block: cell c, other-index combination o of the chunk
+  const int blk = threadIdx.x;
+  const bool mine = blk < b.ncell * C;
+  const int c = mine ? blk / C : 0;
+  const int o = blk - (blk / C) * C;
+  const int ob = ds * o;
+  const int gc = mine ? a.cells[b.cell0 + c] : -1;
+  const bool active = gc >= 0 && gc < a.active_cells;
+  const int64_t gbase = active ? int64_t(gc) * a.N + a.offs[b.off + o] : 0;
+  auto gidx = [&](int si) -> int64_t {
+    const int s0 = (ND == 1) ? si : si / NT;
+    const int s1 = (ND == 1) ? 0 : si % NT;
+    return gbase + int64_t(s0) * b.str[0] + ((ND == 1) ? 0 : int64_t(s1) * b.str[1]);
+  };
+  if (b.resident)
+    gs_copy<TPB>(lists, a.lists + b.list0, b.llen[0] + (ND == 2 ? b.llen[1] : 0));
+  P xv[ds];
+#pragma unroll
+  for (int si = 0; si < ds; si++)
+    xv[si] = active ? x[gidx(si)] : P{0};
+  if (b.resident)
+    gs_copy<TPB>(tab, ctab + b.cb, b.ntab * nq * nn);
+  if (mine)
+  {
+#pragma unroll
+    for (int si = 0; si < ds; si++)
+      X[c * D + ob + si] = xv[si];
+  }
+  __syncthreads(); // the pass list is read before the first pass's barrier
+
+  P acc[ds];
+#pragma unroll
+  for (int si = 0; si < ds; si++)
+    acc[si] = 0;
+  int cur_tab = -1, cur_j = -1;
+  for (int pi = 0; pi < b.npass; pi++)
+  {
+    const gs_pass ps = pass[pi];
+    if (!b.resident && (ps.tab != cur_tab || ps.j != cur_j))
+    {
+      // every earlier pass is done with the staged table and lists
+      __syncthreads();
+      if (ps.j != cur_j)
+        gs_copy<TPB>(lists, a.lists + b.list0 + (ps.j == 0 ? 0 : b.llen[0]),
+                     ps.j == 0 ? b.llen[0] : b.llen[1]);
+      if (ps.tab != cur_tab)
+        gs_copy<TPB>(tab, ctab + b.cb + int64_t(ps.tab) * nq * nn, nq * nn);
+      cur_tab = ps.tab;
+      cur_j = ps.j;
+    }
+    __syncthreads();
+    if (mine)
+    {
+      const P *tb = b.resident ? tab + int64_t(ps.tab) * nq * nn : tab;
+      const int *st = (b.resident && ps.j == 1) ? lists + b.llen[0] : lists;
+      const int *dg = st + b.ncell + 1;
+      const unsigned *ent = reinterpret_cast<const unsigned *>(dg + b.ncell);
+      const P *in = (ps.mode == 2) ? Z : X;
+      P v[ds];
+#pragma unroll
+      for (int si = 0; si < ds; si++)
+        v[si] = 0;
+      if (ND == 1 || ps.j == 0)
+        gs_pass_block<P, NT, ND, 0>(in, tb, st, dg, ent, c, ob, ps.fill, v);
+      else
+        gs_pass_block<P, NT, ND, 1>(in, tb, st, dg, ent, c, ob, ps.fill, v);
+#pragma unroll
+      for (int si = 0; si < ds; si++)
+      {
+        if (ps.mode == 1)
+          Z[c * D + ob + si] = v[si];
+        else
+          acc[si] += v[si];
+      }
+    }
+    // the next pass may overwrite what this one read, or read what it wrote
+    __syncthreads();
+  }
+  if (active)
+  {
+    P *out = work + int64_t(b.slot) * a.stride;
+#pragma unroll
+    for (int si = 0; si < ds; si++)
+      out[gidx(si)] = acc[si];
+  }
+}
+
+//! y = (alpha * sum of the sources + beta*y + gamma*xg) / (1 - jdt * pc) (no
+//! division when pc is null).  The sources, in order: x (src < 0, a term of
+//! identity factors) or work-buffer slot src (a chain's last stage in chain
+//! order, or a slab group's sum at the place of the group's first chain).
 template<typename P>
 __global__ void __launch_bounds__(gk_threads)
-    gk_sum(gk_args a, const P *__restrict__ x, const P *__restrict__ work, P alpha, P beta, P *y,
-           P gamma, const P *xg, const double *pc, double jdt)
+    gk_sum(gk_args a, const int *__restrict__ src, int nsrc, const P *__restrict__ x,
+           const P *__restrict__ work, P alpha, P beta, P *y, P gamma, const P *xg,
+           const double *pc, double jdt)
 {
   const int64_t dofs = a.active_cells * a.N;
   for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < dofs;
        i += int64_t(gk_threads) * gridDim.x)
   {
     P sum = 0;
-    for (int c = 0; c < a.nchains; c++)
+    for (int c = 0; c < nsrc; c++)
     {
-      const int ns = a.ch_nst[c];
-      sum += (ns == 0) ? x[i]
-                       : work[(int64_t((ns - 1) & 1) * a.nslots + a.ch_slot[c]) * a.stride + i];
+      const int sl = src[c];
+      sum += (sl < 0) ? x[i] : work[int64_t(sl) * a.stride + i];
     }
     P out = alpha * sum;
     if (beta != P{0})
@@ -540,6 +886,49 @@ struct akm_global_s
   // lanes per DOF row: 1 (coalesced tensors), 8 (split lists), or 0: one
   // wavefront per cell (n^d <= 64)
   int lanes = 1;
+  // summation sources of gk_sum: chain layouts, and the slab layout
+  akm::dev_buffer<int> src_chain, src_slab;
+  int nsrc_chain = 0, nsrc_slab = 0;
+  // slab layout (layout 6): groups of terms with the same one or two
+  // non-identity dimensions, their bundles and work items; the other chains
+  // run through the cell-layout stages
+  bool slab = false; // chosen by global_autotune or akm_set_tuning
+  int ngroups = 0;
+  akm::dev_buffer<int> gs_passes, gs_tabs, gs_items, gs_cells, gs_lists, gs_offs, gs_qmap;
+  // compact factor tables (per level bound, every group table), rebuilt from
+  // vA at each slab apply by gs_tables; gs_lv: qmap starts then table starts
+  // per level bound (gs_levels + 1 each)
+  akm::dev_buffer<unsigned char> gs_ctab;
+  akm::dev_buffer<int> gs_lv;
+  int64_t gs_ctab_count = 0;
+  int gs_levels = 0;
+  struct gs_launch
+  {
+    int tpb;     // 256 or 1024 threads
+    int nd;      // slab dimensions of its groups
+    bool side;   // on the side stream (split or 1024-thread items)
+    int64_t lds; // dynamic LDS bytes (largest item of the launch)
+    int first, count; // its items
+  };
+  std::vector<gs_launch> gs_launches;
+  // the 1024-thread launch (one large slab per workgroup, few workgroups)
+  // runs beside the others on a stream of its own, forked and joined by events
+  int gs_nparts = 0; // partial slots of split items (after the group slots)
+  int gs_nred = 0;    // cells gs_reduce sums
+  akm::dev_buffer<int> gs_red; // (cell, group slot, chains) per reduced cell
+  hipStream_t gs_side = nullptr;
+  hipEvent_t gs_fork = nullptr, gs_join = nullptr;
+  ~akm_global_s()
+  {
+    if (gs_side)
+      (void)hipStreamDestroy(gs_side);
+    if (gs_fork)
+      (void)hipEventDestroy(gs_fork);
+    if (gs_join)
+      (void)hipEventDestroy(gs_join);
+  }
+  std::vector<akm::dev_buffer<int>> stage_list_rest; // per stage: chains outside the groups
+  std::vector<int> stage_count_rest;
   akm::dev_buffer<unsigned char> work;
   akm::dev_buffer<double> pc; // Jacobi diagonal (active DOFs)
   std::vector<int> c1_pntr_h, c1_indx_h, c1_diag_h;
@@ -582,6 +971,159 @@ static gk_args global_args(akm_matrix_s const &m)
   return a;
 }
 
+//! Launches one slab work-item class: `count` items from `items`.
+template<typename P, int TPB>
+static int gs_launch_class(hipStream_t stream, int n, int nd, gs_args const &ga, const P *ctab,
+                           const gs_item *items, int count, int64_t lds, const P *x, P *work)
+{
+  auto go = [&](auto kern) -> int {
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      return set_error(AKM_ERR_HIP, "global kron slab: %lld bytes of LDS refused",
+                       static_cast<long long>(lds));
+    }
+    kern<<<count, TPB, static_cast<size_t>(lds), stream>>>(ga, ctab, items, x, work);
+    return check_launch("global kron slab");
+  };
+  switch (n * 4 + nd)
+  {
+  case 2 * 4 + 1:
+    return go(gk_slab<P, 2, 1, TPB>);
+  case 2 * 4 + 2:
+    return go(gk_slab<P, 2, 2, TPB>);
+  case 3 * 4 + 1:
+    return go(gk_slab<P, 3, 1, TPB>);
+  case 3 * 4 + 2:
+    return go(gk_slab<P, 3, 2, TPB>);
+  case 4 * 4 + 1:
+    return go(gk_slab<P, 4, 1, TPB>);
+  case 4 * 4 + 2:
+    return go(gk_slab<P, 4, 2, TPB>);
+  default:
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron slab: n = %d", n);
+  }
+}
+
+//! Compact factor tables: entry w of block qc of group table t at level
+//! bound Lp is entry w of block qmap[qm0[Lp] + qc] of the (term, dimension)
+//! tabs[t] of vA.  lv: qm0 (levels + 1 entries), then the table starts
+//! cbase (levels + 1 entries).
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gs_tables(const P *__restrict__ vA, const int *__restrict__ tabs, int ntab,
+              const int *__restrict__ qmap, const int *__restrict__ lv, int levels, int nn,
+              int64_t tstride, int64_t cstride, int64_t count, P *ctab)
+{
+  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < count;
+       i += int64_t(gk_threads) * gridDim.x)
+  {
+    int Lp = 0;
+    while (Lp + 1 < levels && i >= lv[levels + 1 + Lp + 1])
+      Lp++;
+    const int64_t rem = i - lv[levels + 1 + Lp];
+    const int64_t per = int64_t(lv[Lp + 1] - lv[Lp]) * nn;
+    const int t = static_cast<int>(rem / per);
+    const int64_t r2 = rem - t * per;
+    const int64_t qc = r2 / nn;
+    const int w = static_cast<int>(r2 - qc * nn);
+    const int q = qmap[lv[Lp] + qc];
+    ctab[i] = vA[int64_t(q) * cstride + int64_t(tabs[2 * t]) * tstride +
+                 int64_t(tabs[2 * t + 1]) * nn + w];
+  }
+}
+
+//! The group sums of split bundles: slot[e] = sum over the chains, in
+//! chain order, of their partial slots (pslot0 + k), for the N entries of
+//! each listed cell (red: cell, group slot, chains).
+template<typename P>
+__global__ void __launch_bounds__(gk_threads)
+    gs_reduce(const int *__restrict__ red, int nred, int N, int64_t stride, int pslot0, P *work)
+{
+  const int64_t total = int64_t(nred) * N;
+  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < total;
+       i += int64_t(gk_threads) * gridDim.x)
+  {
+    const int64_t r = i / N;
+    const int64_t at = int64_t(red[3 * r]) * N + (i - r * N);
+    const int nch = red[3 * r + 2];
+    P sum = work[int64_t(pslot0) * stride + at];
+    for (int k = 1; k < nch; k++)
+      sum += work[int64_t(pslot0 + k) * stride + at];
+    work[int64_t(red[3 * r + 1]) * stride + at] = sum;
+  }
+}
+
+template<typename P>
+static int gs_reduce_launch(akm_matrix_s &m, hipStream_t stream, P *work)
+{
+  akm_global_s &g = *m.gk;
+  const int64_t total = int64_t(g.gs_nred) * m.N;
+  gs_reduce<P><<<gk_blocks(total), gk_threads, 0, stream>>>(
+      g.gs_red.data(), g.gs_nred, static_cast<int>(m.N), g.cells * m.N, 2 * g.nslots + g.ngroups,
+      work);
+  return check_launch("global kron slab reduce");
+}
+
+//! Every slab group's sum into its work-buffer slot.
+template<typename P>
+static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
+{
+  akm_global_s &g = *m.gk;
+  gs_args ga{};
+  ga.d = m.d;
+  ga.n = m.n;
+  ga.N = static_cast<int>(m.N);
+  ga.T = m.T;
+  ga.active_cells = g.active_cells;
+  ga.stride = g.cells * m.N;
+  ga.passes = reinterpret_cast<const gs_pass *>(g.gs_passes.data());
+  ga.cells = g.gs_cells.data();
+  ga.lists = g.gs_lists.data();
+  ga.offs = g.gs_offs.data();
+  // the compact factor tables from the current vA
+  const P *ctab = reinterpret_cast<const P *>(g.gs_ctab.data());
+  {
+    const int nn = m.n * m.n;
+    const int64_t tstride = int64_t(m.d) * nn;
+    gs_tables<P><<<gk_blocks(g.gs_ctab_count), gk_threads, 0, m.stream>>>(
+        vA, g.gs_tabs.data(), static_cast<int>(g.gs_tabs.size() / 2), g.gs_qmap.data(),
+        g.gs_lv.data(), g.gs_levels, nn, tstride, tstride * m.T, g.gs_ctab_count,
+        reinterpret_cast<P *>(g.gs_ctab.data()));
+    AKM_TRY(check_launch("global kron slab tables"));
+  }
+  // the side launches come first and run on the side stream; gs_reduce
+  // follows the join
+  const bool fork = g.gs_side != nullptr;
+  if (fork)
+  {
+    AKM_HIP(hipEventRecord(g.gs_fork, m.stream));
+    AKM_HIP(hipStreamWaitEvent(g.gs_side, g.gs_fork, 0));
+  }
+  const hipStream_t side = fork ? g.gs_side : m.stream;
+  for (size_t li = 0; li < g.gs_launches.size(); li++)
+  {
+    auto const &l = g.gs_launches[li];
+    const hipStream_t s = l.side ? side : m.stream;
+    const gs_item *items = reinterpret_cast<const gs_item *>(g.gs_items.data()) + l.first;
+    if (l.tpb == 1024)
+      AKM_TRY((gs_launch_class<P, 1024>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work)));
+    else
+      AKM_TRY((gs_launch_class<P, 256>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work)));
+  }
+  if (fork)
+  {
+    AKM_HIP(hipEventRecord(g.gs_join, g.gs_side));
+    AKM_HIP(hipStreamWaitEvent(m.stream, g.gs_join, 0));
+  }
+  if (g.gs_nred > 0)
+    AKM_TRY(gs_reduce_launch<P>(m, m.stream, work));
+  return AKM_SUCCESS;
+}
+
 template<typename P>
 static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y, P gamma,
                               const P *xg, bool jacobi, double jdt)
@@ -599,15 +1141,18 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
   const gk_args a = global_args(m);
   const P *vA = reinterpret_cast<const P *>(m.vA.data());
   P *work = reinterpret_cast<P *>(g.work.data());
-  const int lanes = g.lanes;
+  // the slab layout runs the chains outside its groups in the cell layout
+  const bool slab = g.slab;
+  const int lanes = slab ? (m.N <= wave_size ? 0 : 1) : g.lanes;
   for (int st = 0; st < g.max_stages; st++)
   {
-    const int nl = g.stage_count[st];
+    const int nl = slab ? g.stage_count_rest[st] : g.stage_count[st];
     if (nl == 0)
       continue;
+    const int *list = slab ? g.stage_list_rest[st].data() : g.stage_list[st].data();
     const int blocks = (lanes == 0) ? gk_blocks_xcd(g.cells * nl * wave_size)
                                     : gk_blocks(g.cells * m.N * nl * lanes);
-    if (lanes == 0 && st == 0 && g.first_grouped && g.first_largest > 1 && m.n >= 2 &&
+    if (!slab && lanes == 0 && st == 0 && g.first_grouped && g.first_largest > 1 && m.n >= 2 &&
         m.n <= 4)
     {
       // chains sharing their first-stage input read it once
@@ -630,7 +1175,6 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
     }
     else if (lanes == 0)
     {
-      const int *list = g.stage_list[st].data();
       switch (m.n)
       {
       case 2:
@@ -647,15 +1191,17 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
       }
     }
     else if (lanes == 8)
-      gk_stage<P, 8><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl, st,
-                                                           x, work);
+      gk_stage<P, 8><<<blocks, gk_threads, 0, m.stream>>>(a, vA, list, nl, st, x, work);
     else
-      gk_stage<P, 1><<<blocks, gk_threads, 0, m.stream>>>(a, vA, g.stage_list[st].data(), nl, st,
-                                                           x, work);
+      gk_stage<P, 1><<<blocks, gk_threads, 0, m.stream>>>(a, vA, list, nl, st, x, work);
     AKM_TRY(check_launch("global kron stage"));
   }
-  gk_sum<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, x, work, alpha, beta, y, gamma, xg,
-                                                          pc, jdt);
+  if (slab)
+    AKM_TRY(gs_apply<P>(m, vA, x, work));
+  const int *src = slab ? g.src_slab.data() : g.src_chain.data();
+  const int nsrc = slab ? g.nsrc_slab : g.nsrc_chain;
+  gk_sum<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, src, nsrc, x, work, alpha, beta, y,
+                                                          gamma, xg, pc, jdt);
   return check_launch("global kron sum");
 }
 
@@ -774,6 +1320,8 @@ int global_layout(akm_matrix_s const &m)
 {
   if (!m.gk)
     return 0;
+  if (m.gk->slab)
+    return 6;
   if (m.gk->lanes == 0)
     return (m.gk->first_grouped ? 3 : 2) + (m.gk->xcd_order ? 2 : 0);
   return m.gk->lanes;
@@ -785,11 +1333,20 @@ int global_set_layout(akm_matrix_s &m, int layout)
     return set_error(AKM_ERR_INVALID, "not a global kron matrix");
   if (layout == 0)
     return AKM_SUCCESS;
-  if (layout != 1 && layout != 8 && (layout < 2 || layout > 5))
-    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2, 3, 4, 5 or 8", layout);
+  if (layout != 1 && layout != 8 && (layout < 2 || layout > 6))
+    return set_error(AKM_ERR_INVALID, "global kron layout %d is not 1, 2, 3, 4, 5, 6 or 8",
+                     layout);
   const bool cell = (layout >= 2 && layout <= 5);
   if (cell && m.N > wave_size)
     return set_error(AKM_ERR_UNSUPPORTED, "global kron: the cell layout needs n^d <= 64");
+  m.gk->slab = (layout == 6);
+  if (m.gk->slab)
+  {
+    m.gk->lanes = 0;
+    m.gk->first_grouped = false;
+    m.gk->xcd_order = false;
+    return AKM_SUCCESS;
+  }
   m.gk->lanes = cell ? 0 : layout;
   m.gk->first_grouped = (layout == 3 || layout == 5);
   m.gk->xcd_order = (layout == 4 || layout == 5);
@@ -808,23 +1365,29 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
   int best_lanes = 1;
   bool best_grouped = false;
   int rc = AKM_SUCCESS;
-  bool best_xcd = false;
+  bool best_xcd = false, best_slab = false;
   // layouts 1 and 8, then the cell layout with and without the grouped
   // first stage (shared input reads against fewer, longer wavefronts), each
-  // cell layout with and without the XCD-contiguous workgroup order
-  const int shapes[6][3] = {{1, 0, 0}, {8, 0, 0}, {0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}};
+  // cell layout with and without the XCD-contiguous workgroup order, then
+  // the slab layout (when some term has one or two non-identity dimensions)
+  const int shapes[7][4] = {{1, 0, 0, 0}, {8, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 1, 0},
+                            {0, 1, 0, 0}, {0, 1, 1, 0}, {0, 0, 0, 1}};
   for (auto const &shape : shapes)
   {
     const int lanes = shape[0];
     const bool grouped = shape[1] != 0;
     const bool xcd = shape[2] != 0;
-    if (lanes == 0 && m.N > wave_size)
+    const bool slab = shape[3] != 0;
+    if (lanes == 0 && !slab && m.N > wave_size)
       continue;
     if (grouped && g.first_largest < 2)
+      continue;
+    if (slab && g.ngroups == 0)
       continue;
     g.lanes = lanes;
     g.first_grouped = grouped;
     g.xcd_order = xcd;
+    g.slab = slab;
     rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
     if (rc != AKM_SUCCESS)
       break;
@@ -840,11 +1403,13 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
       best_lanes = lanes;
       best_grouped = grouped;
       best_xcd = xcd;
+      best_slab = slab;
     }
   }
   g.lanes = best_lanes;
   g.first_grouped = best_grouped;
   g.xcd_order = best_xcd;
+  g.slab = best_slab;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
@@ -861,6 +1426,513 @@ static void parents_1d(int i, std::vector<int> &out)
     i = (i == 1) ? 0 : i / 2;
     out.push_back(i);
   }
+}
+
+//! Level of a 1D cell index (0 for cell 0, l for 2^(l-1) <= i < 2^l).
+static int level_1d(int i)
+{
+  int l = 0;
+  while (i > 0)
+  {
+    i >>= 1;
+    l++;
+  }
+  return l;
+}
+
+//! Per-dimension host copies of the fibre connection lists.
+struct gk_lists
+{
+  std::vector<int> ptr, cell, q, diag;
+};
+
+//! One bundle while the slab plan is made.
+struct gs_plan_bundle
+{
+  std::vector<int> cells;
+  int C = 1, Lp = 0, tpb = 256;
+};
+
+//! Builds the slab layout: groups of the terms with one or two non-identity
+//! dimensions (act[t] after the flux swap), their slabs, bundles, lists and
+//! work items.  A group whose largest slab does not fit one workgroup at
+//! one other-index combination per chunk is left to the cell-layout stages.
+static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> const &lists,
+                       std::vector<std::vector<int>> const &term_act,
+                       std::vector<int> const &ch_term, std::vector<int> const &ch_nst,
+                       std::vector<int> const &ch_dim, std::vector<int> const &ch_fill,
+                       std::vector<int> const &ch_slot)
+{
+  const int d = m.d, n = m.n, T = m.T;
+  const int L = m.max_level;
+  const int nn = n * n;
+  const int esize = static_cast<int>(m.esize);
+  // compact factor index per level bound Lp: connect_1d entries with row and
+  // column below 2^Lp, in connect_1d order
+  std::vector<std::vector<int>> qc_of(L + 1);
+  std::vector<int> qmap, qm0;
+  for (int Lp = 0; Lp <= L; Lp++)
+  {
+    qm0.push_back(static_cast<int>(qmap.size()));
+    qc_of[Lp].assign(g.c1_indx_h.size(), -1);
+    const int lim = 1 << Lp;
+    int cnt = 0;
+    for (int r = 0; r < lim && r + 1 < static_cast<int>(g.c1_pntr_h.size()); r++)
+      for (int q = g.c1_pntr_h[r]; q < g.c1_pntr_h[r + 1]; q++)
+        if (g.c1_indx_h[q] < lim)
+        {
+          qc_of[Lp][q] = cnt++;
+          qmap.push_back(q);
+        }
+  }
+  qm0.push_back(static_cast<int>(qmap.size()));
+
+  // groups by the set of non-identity dimensions
+  std::vector<int> term_group(T, -1);
+  std::vector<std::vector<int>> gdims;
+  for (int t = 0; t < T; t++)
+  {
+    const int na = static_cast<int>(term_act[t].size());
+    if (na < 1 || na > 2)
+      continue;
+    std::vector<int> key = term_act[t];
+    std::sort(key.begin(), key.end());
+    size_t gi = 0;
+    while (gi < gdims.size() && gdims[gi] != key)
+      gi++;
+    if (gi == gdims.size())
+      gdims.push_back(key);
+    term_group[t] = static_cast<int>(gi);
+  }
+
+  std::vector<int> bundles, groups, passes, tabs, items, cells, lists_img, offs;
+  struct item_rec
+  {
+    int bundle, h, tpb, nd;
+    int64_t lds;
+    int resident;
+    int chain = -1;          // >= 0: this chain of the group only (a split item)
+    int Lp = 0;              // level bound of its bundle
+    int pass_off = 0, pass_cnt = 0; // its passes within the group's
+  };
+  std::vector<item_rec> irec;
+  std::vector<int> loc(g.cells, -1);
+  std::vector<int> gslot(gdims.size(), -1);
+  int ngroups = 0;
+  for (size_t gi = 0; gi < gdims.size(); gi++)
+  {
+    const std::vector<int> &S = gdims[gi];
+    const int nd = static_cast<int>(S.size());
+    const int ds = (nd == 1) ? n : nn;
+    const int64_t gs_cap_small = int64_t(gs_blocks_small) * ds; // values
+    const int64_t gs_cap_large = int64_t(gs_blocks_large) * ds;
+    if (n < 2 || n > 4)
+      continue; // the slab kernels hold n^nd-value blocks in registers
+    int call = 1;
+    for (int k = nd; k < d; k++)
+      call *= n;
+    // slabs: cells equal in every 1D index outside S, cell order kept
+    std::map<std::vector<int>, std::vector<int>> slabs;
+    std::vector<int> key;
+    for (int64_t c = 0; c < g.cells; c++)
+    {
+      key.clear();
+      for (int k = 0; k < d; k++)
+        if (std::find(S.begin(), S.end(), k) == S.end())
+          key.push_back(g.elem_host[c * d + k]);
+      slabs[key].push_back(static_cast<int>(c));
+    }
+    // bundles: small slabs packed up to gs_cap_small values, a larger slab
+    // alone with its other-index combinations split into chunks
+    std::vector<gs_plan_bundle> plan;
+    bool ok = true;
+    gs_plan_bundle open;
+    int64_t open_vals = 0;
+    auto slab_level = [&](std::vector<int> const &sc) {
+      int lp = 0;
+      for (int c : sc)
+        for (int k : S)
+          lp = std::max(lp, level_1d(g.elem_host[int64_t(c) * d + k]));
+      return lp;
+    };
+    for (auto const &kv : slabs)
+    {
+      const std::vector<int> &sc = kv.second;
+      const int64_t msz = static_cast<int64_t>(sc.size());
+      const int64_t full = msz * ds * call;
+      const int lp = slab_level(sc);
+      if (full <= gs_cap_small)
+      {
+        if (open_vals + full > gs_cap_small)
+        {
+          plan.push_back(open);
+          open = gs_plan_bundle();
+          open_vals = 0;
+        }
+        open.cells.insert(open.cells.end(), sc.begin(), sc.end());
+        open.C = call;
+        open.Lp = std::max(open.Lp, lp);
+        open_vals += full;
+        continue;
+      }
+      const int64_t cap = (msz * ds <= gs_cap_small) ? gs_cap_small : gs_cap_large;
+      if (msz * ds > cap)
+      {
+        ok = false;
+        break;
+      }
+      gs_plan_bundle one;
+      one.cells = sc;
+      one.Lp = lp;
+      one.tpb = (cap == gs_cap_small) ? 256 : 1024;
+      one.C = call;
+      while (msz * ds * one.C > cap)
+        one.C /= n;
+      plan.push_back(one);
+    }
+    if (ok && !open.cells.empty())
+      plan.push_back(open);
+    if (!ok)
+      continue;
+
+    // the group's terms in chain order: factor tables (term, S[j]) at
+    // term index * nd + j
+    std::vector<int> gterms;
+    for (size_t c = 0; c < ch_term.size(); c++)
+      if (term_group[ch_term[c]] == static_cast<int>(gi) &&
+          std::find(gterms.begin(), gterms.end(), ch_term[c]) == gterms.end())
+        gterms.push_back(ch_term[c]);
+    const int ntab = static_cast<int>(gterms.size()) * nd;
+    int npass = 0;
+    std::vector<std::pair<int, int>> gchains; // (first pass, passes) per chain
+    for (size_t c = 0; c < ch_term.size(); c++)
+      if (term_group[ch_term[c]] == static_cast<int>(gi))
+      {
+        gchains.emplace_back(npass, ch_nst[c]);
+        npass += ch_nst[c];
+      }
+
+    // lists, LDS sizes and items of the plan (checked before anything is kept)
+    const int group_index = ngroups;
+    std::vector<int> b_rec, p_cells, p_lists;
+    std::vector<item_rec> p_items;
+    const int bundle_base = static_cast<int>(bundles.size() / (sizeof(gs_bundle) / sizeof(int)));
+    for (size_t bi = 0; bi < plan.size() && ok; bi++)
+    {
+      gs_plan_bundle const &pb = plan[bi];
+      const int ncell = static_cast<int>(pb.cells.size());
+      gs_bundle rec{};
+      rec.group = group_index;
+      rec.cell0 = static_cast<int>(cells.size() + p_cells.size());
+      rec.ncell = ncell;
+      rec.C = pb.C;
+      rec.Lp = pb.Lp;
+      rec.list0 = static_cast<int>(lists_img.size() + p_lists.size());
+      for (int c = 0; c < ncell; c++)
+        loc[pb.cells[c]] = c;
+      int nent_max = 0, nent_sum = 0;
+      for (int j = 0; j < nd && ok; j++)
+      {
+        // image: ncell + 1 starts, ncell diagonal positions, the entries
+        gk_lists const &ls = lists[S[j]];
+        const size_t img = p_lists.size();
+        p_lists.resize(img + 2 * size_t(ncell) + 1);
+        int cnt = 0;
+        for (int c = 0; c < ncell && ok; c++)
+        {
+          const int gc = pb.cells[c];
+          p_lists[img + c] = cnt;
+          p_lists[img + ncell + 1 + c] = cnt + ls.diag[gc] - ls.ptr[gc];
+          for (int p = ls.ptr[gc]; p < ls.ptr[gc + 1]; p++)
+          {
+            // element offset of the connected cell in x / the intermediate,
+            // element offset of its factor block in the table
+            const int64_t xo = int64_t(loc[ls.cell[p]]) * ds * pb.C;
+            const int64_t ao = int64_t(qc_of[pb.Lp][ls.q[p]]) * nn;
+            if (xo < 0 || ao < 0 || xo > 0xffff || ao > 0xffff)
+            {
+              ok = false; // outside the slab (cannot happen) or past 16-bit offsets
+              break;
+            }
+            p_lists.push_back(static_cast<int>(static_cast<unsigned>(xo) |
+                                               (static_cast<unsigned>(ao) << 16)));
+            cnt++;
+          }
+        }
+        p_lists[img + ncell] = cnt;
+        rec.llen[j] = static_cast<int>(p_lists.size() - img);
+        nent_max = std::max(nent_max, cnt);
+        nent_sum += cnt;
+      }
+      for (int c = 0; c < ncell; c++)
+        loc[pb.cells[c]] = -1;
+      if (!ok)
+        break;
+      const int nq = qm0[pb.Lp + 1] - qm0[pb.Lp];
+      // everything staged once when that keeps two workgroups per CU, else
+      // one table and one list at a time
+      int64_t lds = gs_lds_bytes(esize, nd, n, ncell, pb.C, nq, ntab,
+                                 int64_t(nd) * (2 * ncell + 1) + nent_sum, npass);
+      rec.resident = (lds <= gs_lds_resident) ? 1 : 0;
+      if (!rec.resident)
+        lds = gs_lds_bytes(esize, nd, n, ncell, pb.C, nq, 1, 2 * ncell + 1 + nent_max, npass);
+      if (lds > gs_lds_max)
+      {
+        ok = false;
+        break;
+      }
+      const int *rp = reinterpret_cast<const int *>(&rec);
+      b_rec.insert(b_rec.end(), rp, rp + sizeof(gs_bundle) / sizeof(int));
+      p_cells.insert(p_cells.end(), pb.cells.begin(), pb.cells.end());
+      // a slab with long connection lists (level bound >= gs_split_level) runs
+      // one workgroup per chain: its passes are the longest
+      const bool split = pb.Lp >= gs_split_level && gchains.size() > 1;
+      for (int h = 0; h < call / pb.C; h++)
+      {
+        if (!split)
+        {
+          p_items.push_back({bundle_base + static_cast<int>(bi), h, pb.tpb, nd, lds,
+                             rec.resident});
+          continue;
+        }
+        for (size_t k = 0; k < gchains.size(); k++)
+        {
+          item_rec r{bundle_base + static_cast<int>(bi), h, pb.tpb, nd,
+                     gs_lds_bytes(esize, nd, n, ncell, pb.C, nq, 1, 2 * ncell + 1 + nent_max,
+                                  gchains[k].second),
+                     0};
+          r.chain = static_cast<int>(k);
+          r.Lp = pb.Lp;
+          r.pass_off = gchains[k].first;
+          r.pass_cnt = gchains[k].second;
+          p_items.push_back(r);
+        }
+      }
+    }
+    if (!ok)
+      continue;
+    // the group is kept
+    gs_group grec{};
+    grec.nd = nd;
+    for (int j = 0; j < nd; j++)
+    {
+      grec.dim[j] = S[j];
+      int str = 1;
+      for (int k = S[j] + 1; k < d; k++)
+        str *= n;
+      grec.str[j] = str;
+    }
+    grec.call = call;
+    grec.off0 = static_cast<int>(offs.size());
+    for (int o = 0; o < call; o++)
+    {
+      // digits of o over the other dimensions, the last one fastest
+      int rem = o, off = 0, str = 1;
+      for (int k = d - 1; k >= 0; k--)
+      {
+        if (std::find(S.begin(), S.end(), k) == S.end())
+        {
+          off += (rem % n) * str;
+          rem /= n;
+        }
+        str *= n;
+      }
+      offs.push_back(off);
+    }
+    grec.tab0 = static_cast<int>(tabs.size() / 2);
+    grec.ntab = ntab;
+    for (int t : gterms)
+      for (int j = 0; j < nd; j++)
+        tabs.insert(tabs.end(), {t, S[j]});
+    grec.pass0 = static_cast<int>(passes.size() / 4);
+    for (size_t c = 0; c < ch_term.size(); c++)
+    {
+      const int t = ch_term[c];
+      if (term_group[t] != static_cast<int>(gi))
+        continue;
+      const int ti = static_cast<int>(std::find(gterms.begin(), gterms.end(), t) -
+                                      gterms.begin());
+      const int ns = ch_nst[c];
+      for (int st = 0; st < ns; st++)
+      {
+        const int k = ch_dim[c * gk_max_dims + st];
+        const int j = (k == S[0]) ? 0 : 1;
+        const gs_pass ps{ti * nd + j, j, ch_fill[c * gk_max_dims + st],
+                         (ns == 1) ? 0 : (st == 0 ? 1 : 2)};
+        passes.insert(passes.end(), {ps.tab, ps.j, ps.fill, ps.mode});
+      }
+    }
+    grec.npass = static_cast<int>(passes.size() / 4) - grec.pass0;
+    grec.slot = 2 * g.nslots + group_index;
+    const int *gp = reinterpret_cast<const int *>(&grec);
+    groups.insert(groups.end(), gp, gp + sizeof(gs_group) / sizeof(int));
+    bundles.insert(bundles.end(), b_rec.begin(), b_rec.end());
+    cells.insert(cells.end(), p_cells.begin(), p_cells.end());
+    lists_img.insert(lists_img.end(), p_lists.begin(), p_lists.end());
+    irec.insert(irec.end(), p_items.begin(), p_items.end());
+    gslot[gi] = group_index;
+    ngroups++;
+  }
+  for (int t = 0; t < T; t++)
+    if (term_group[t] >= 0 && gslot[term_group[t]] < 0)
+      term_group[t] = -1; // its group did not fit: cell-layout stages
+  g.ngroups = ngroups;
+
+  // launches: 1024-thread items, then 256-thread items above and below 32 KB
+  // of LDS (the small ones keep more workgroups per CU)
+  // the side stream takes the 1024-thread items and the split items of the
+  // largest slabs (level bound >= gs_side_level), about half the work on
+  // the Vlasov operator; they come first
+  auto on_side = [](item_rec const &r) {
+    return r.tpb == 1024 || (r.chain >= 0 && r.Lp >= gs_side_level);
+  };
+  auto cls = [&](item_rec const &r) {
+    const int side = on_side(r) ? 0 : 1;
+    return ((side * 2 + (r.tpb == 1024 ? 0 : 1)) * 2 + (r.lds > 32 * 1024 ? 0 : 1)) * 2 +
+           (r.nd - 1);
+  };
+  // within a launch the largest items first (the longest-running workgroups
+  // start early)
+  std::stable_sort(irec.begin(), irec.end(), [&](item_rec const &u, item_rec const &v) {
+    return cls(u) != cls(v) ? cls(u) < cls(v) : u.lds > v.lds;
+  });
+  g.gs_launches.clear();
+  int nparts = 0;
+  // compact factor tables, per level bound every group table: cbase[Lp]
+  const int64_t ntab_all = static_cast<int64_t>(tabs.size() / 2);
+  std::vector<int64_t> cbase(L + 2, 0);
+  for (int Lp = 0; Lp <= L; Lp++)
+    cbase[Lp + 1] = cbase[Lp] + ntab_all * (qm0[Lp + 1] - qm0[Lp]) * nn;
+  if (cbase[L + 1] > (int64_t(1) << 31) - 1)
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: slab factor tables too large");
+  const gs_bundle *hb = reinterpret_cast<const gs_bundle *>(bundles.data());
+  const gs_group *hg = reinterpret_cast<const gs_group *>(groups.data());
+  for (size_t i = 0; i < irec.size(); i++)
+  {
+    const gs_bundle &bb = hb[irec[i].bundle];
+    const gs_group &gg = hg[bb.group];
+    gs_item it{};
+    it.cell0 = bb.cell0;
+    it.ncell = bb.ncell;
+    it.C = bb.C;
+    it.nq = qm0[bb.Lp + 1] - qm0[bb.Lp];
+    it.cb = static_cast<int>(cbase[bb.Lp] + int64_t(gg.tab0) * it.nq * nn);
+    it.resident = irec[i].resident;
+    it.list0 = bb.list0;
+    it.llen[0] = bb.llen[0];
+    it.llen[1] = bb.llen[1];
+    it.nd = gg.nd;
+    it.str[0] = gg.str[0];
+    it.str[1] = gg.str[1];
+    it.off = gg.off0 + irec[i].h * bb.C;
+    it.tab0 = gg.tab0;
+    it.ntab = gg.ntab;
+    const bool split = irec[i].chain >= 0;
+    it.pass0 = gg.pass0 + (split ? irec[i].pass_off : 0);
+    it.npass = split ? irec[i].pass_cnt : gg.npass;
+    // a split item writes its chain's partial slot, summed by gs_reduce
+    it.slot = split ? 2 * g.nslots + ngroups + irec[i].chain : gg.slot;
+    if (split)
+      nparts = std::max(nparts, irec[i].chain + 1);
+    const int *ip = reinterpret_cast<const int *>(&it);
+    items.insert(items.end(), ip, ip + sizeof(gs_item) / sizeof(int));
+    if (g.gs_launches.empty() || cls(irec[i - 1]) != cls(irec[i]))
+      g.gs_launches.push_back({irec[i].tpb, irec[i].nd, on_side(irec[i]), 0,
+                               static_cast<int>(i), 0});
+    auto &l = g.gs_launches.back();
+    l.lds = std::max(l.lds, irec[i].lds);
+    l.count++;
+  }
+  // gs_reduce: the active cells of split bundles, their group slot and chain
+  // count (every chunk of a split bundle is split, so each cell once)
+  std::vector<int> red;
+  {
+    std::vector<bool> seen(bundles.size() / (sizeof(gs_bundle) / sizeof(int)), false);
+    for (auto const &r : irec)
+    {
+      if (r.chain < 0 || seen[r.bundle])
+        continue;
+      seen[r.bundle] = true;
+      const gs_bundle &bb = hb[r.bundle];
+      const gs_group &gg = hg[bb.group];
+      int nch = 0;
+      for (auto const &r2 : irec)
+        if (r2.bundle == r.bundle && r2.h == r.h)
+          nch++;
+      for (int c = 0; c < bb.ncell; c++)
+      {
+        const int gc = cells[bb.cell0 + c];
+        if (gc < g.active_cells)
+          red.insert(red.end(), {gc, gg.slot, nch});
+      }
+    }
+  }
+  g.gs_nparts = nparts;
+  g.gs_nred = static_cast<int>(red.size() / 3);
+  if (!red.empty())
+    AKM_TRY(g.gs_red.upload(red.data(), int64_t(red.size())));
+  if (!g.gs_launches.empty() && g.gs_launches[0].side && !g.gs_launches.back().side &&
+      g.gs_side == nullptr)
+  {
+    AKM_HIP(hipStreamCreateWithFlags(&g.gs_side, hipStreamNonBlocking));
+    AKM_HIP(hipEventCreateWithFlags(&g.gs_fork, hipEventDisableTiming));
+    AKM_HIP(hipEventCreateWithFlags(&g.gs_join, hipEventDisableTiming));
+  }
+  if (ngroups > 0)
+  {
+    AKM_TRY(g.gs_passes.upload(passes.data(), int64_t(passes.size())));
+    AKM_TRY(g.gs_tabs.upload(tabs.data(), int64_t(tabs.size())));
+    AKM_TRY(g.gs_items.upload(items.data(), int64_t(items.size())));
+    AKM_TRY(g.gs_cells.upload(cells.data(), int64_t(cells.size())));
+    AKM_TRY(g.gs_lists.upload(lists_img.data(), int64_t(lists_img.size())));
+    AKM_TRY(g.gs_offs.upload(offs.data(), int64_t(offs.size())));
+    AKM_TRY(g.gs_qmap.upload(qmap.data(), int64_t(qmap.size())));
+    std::vector<int> lv(qm0.begin(), qm0.end());
+    for (int64_t v : cbase)
+      lv.push_back(static_cast<int>(v));
+    AKM_TRY(g.gs_lv.upload(lv.data(), int64_t(lv.size())));
+    AKM_TRY(g.gs_ctab.allocate(cbase[L + 1] * esize));
+    g.gs_ctab_count = cbase[L + 1];
+    g.gs_levels = L + 1;
+  }
+
+  // the chains outside the groups, per stage, and the summation sources
+  g.stage_list_rest.clear();
+  g.stage_list_rest.resize(g.max_stages);
+  g.stage_count_rest.assign(g.max_stages, 0);
+  for (int st = 0; st < g.max_stages; st++)
+  {
+    std::vector<int> list;
+    for (size_t c = 0; c < ch_term.size(); c++)
+      if (term_group[ch_term[c]] < 0 && ch_nst[c] >= st + 1)
+        list.push_back(static_cast<int>(c));
+    g.stage_count_rest[st] = static_cast<int>(list.size());
+    if (!list.empty())
+      AKM_TRY(g.stage_list_rest[st].upload(list.data(), int64_t(list.size())));
+  }
+  std::vector<int> src_chain, src_slab;
+  std::vector<bool> emitted(gdims.size(), false);
+  for (size_t c = 0; c < ch_term.size(); c++)
+  {
+    const int ns = ch_nst[c];
+    const int own = (ns == 0) ? -1 : ((ns - 1) & 1) * g.nslots + ch_slot[c];
+    src_chain.push_back(own);
+    const int gi = term_group[ch_term[c]];
+    if (gi < 0)
+      src_slab.push_back(own);
+    else if (!emitted[gi])
+    {
+      emitted[gi] = true;
+      src_slab.push_back(2 * g.nslots + gslot[gi]);
+    }
+  }
+  g.nsrc_chain = static_cast<int>(src_chain.size());
+  g.nsrc_slab = static_cast<int>(src_slab.size());
+  if (!src_chain.empty())
+    AKM_TRY(g.src_chain.upload(src_chain.data(), int64_t(src_chain.size())));
+  if (!src_slab.empty())
+    AKM_TRY(g.src_slab.upload(src_slab.data(), int64_t(src_slab.size())));
+  return AKM_SUCCESS;
 }
 
 //! Builds the padded cell set, the fibres, the chains and the work buffers
@@ -925,6 +1997,7 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
   // per dimension: the fibres (cells equal in every other 1D index) and for
   // every cell its connected cells of the fibre, 1D index ascending
   std::vector<int64_t> nnz_dim(d, 0); // DOF-level pattern entries per dimension
+  std::vector<gk_lists> lists(d);
   for (int k = 0; k < d; k++)
   {
     std::map<std::vector<int>, int> fib;
@@ -971,6 +2044,10 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
     AKM_TRY(g->nb_cell[k].upload(ncell.data(), int64_t(ncell.size())));
     AKM_TRY(g->nb_q[k].upload(nq.data(), int64_t(nq.size())));
     AKM_TRY(g->nb_diag[k].upload(ndiag.data(), int64_t(ndiag.size())));
+    lists[k].ptr = std::move(ptr);
+    lists[k].cell = std::move(ncell);
+    lists[k].q = std::move(nq);
+    lists[k].diag = std::move(ndiag);
     // processing order: fibre by fibre (fibres in key order), 1D index ascending
     std::vector<int> ord;
     ord.reserve(g->cells);
@@ -983,6 +2060,7 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
   // chains (kronmult::permutes, src/device/asgard_kronmult.hpp:152-220, and
   // the active-direction order of src/asgard_kronmult_matrix.cpp:1313-1331)
   std::vector<int> ch_term, ch_nst, ch_dim, ch_fill, ch_slot;
+  std::vector<std::vector<int>> term_act(m.T);
   for (int t = 0; t < m.T; t++)
   {
     std::vector<int> act;
@@ -997,6 +2075,7 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
       if (fd >= 0 && fd < na && fd != act[0])
         std::swap(act[0], act[fd]);
     }
+    term_act[t] = act;
     if (na == 0)
     {
       ch_term.push_back(t);
@@ -1097,8 +2176,10 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
     AKM_TRY(g->first_ptr.upload(gptr.data(), int64_t(gptr.size())));
     AKM_TRY(g->first_chain.upload(gchain.data(), int64_t(gchain.size())));
   }
-  if (nslots > 0)
-    AKM_TRY(g->work.allocate(int64_t(2) * nslots * g->cells * m.N * int64_t(m.esize)));
+  AKM_TRY(build_slabs(m, *g, lists, term_act, ch_term, ch_nst, ch_dim, ch_fill, ch_slot));
+  const int64_t work_slots = int64_t(2) * nslots + g->ngroups + g->gs_nparts;
+  if (work_slots > 0)
+    AKM_TRY(g->work.allocate(work_slots * g->cells * m.N * int64_t(m.esize)));
   // flops as the reference counts them (src/asgard_kronmult_matrix.cpp:
   // 1389-1395): two per entry of the full DOF-level pattern of every
   // non-identity (term, dimension) on the padded set

@@ -183,7 +183,8 @@ akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_
                            int64_t *output_size, int64_t *num_pairs, int64_t *flops);
 
 /* Tuning knobs: wavefronts per row workgroup (0 = automatic, else 1, 2, 4,
- * 8 or 16; 16 only for the matrix-core kernels, the others run 8), the LDS bytes a workgroup may hold for the row's factor tables
+ * 8 or 16; 16 only for the matrix-core kernels, the others run 8), the LDS
+ * bytes a workgroup may hold for the row's factor tables
  * (-1 = automatic, 0 = read factors from global memory) and whether to
  * force the block-wide LDS kernel (for tests).  Takes effect at the next
  * apply.  On a global operator only waves_per_row counts and picks the row
@@ -191,8 +192,12 @@ akm_status akm_matrix_info(akm_matrix mat, int64_t *tensor_size, int64_t *input_
  * per cell (n^d <= 64), 3 = as 2 with the chains whose first stage reads
  * the same input along the same dimension grouped (one read for up to four
  * chains), 4 / 5 = as 2 / 3 with the workgroups of each XCD taking
- * contiguous cells (neighbours shared in one L2), 0 = unchanged;
- * akm_autotune times them and reports its choice the same way.  No counterpart in the reference (its launch
+ * contiguous cells (neighbours shared in one L2), 6 = slabs: the terms
+ * with one or two non-identity dimensions grouped by those dimensions, one
+ * workgroup per bundle of slabs (cells equal outside the group's
+ * dimensions) running all the group's chains in LDS, the other terms as 2
+ * (or 1 when n^d > 64), 0 = unchanged; akm_autotune times them and reports
+ * its choice the same way.  No counterpart in the reference (its launch
  * shape is fixed at build time, CMakeLists.txt:231-232). */
 akm_status akm_set_tuning(akm_matrix mat, int waves_per_row, int lds_cap_bytes,
                           int force_generic);

@@ -74,8 +74,9 @@ def _case_and_refs(spec):
 
 
 # lane per DOF, 8 lanes per DOF, wavefront per cell (3: grouped first stage;
-# 4, 5: 2, 3 with XCD-contiguous workgroups)
-LAYOUTS = (1, 8, 2, 3, 4, 5)
+# 4, 5: 2, 3 with XCD-contiguous workgroups), 6: slabs (terms with one or two
+# non-identity dimensions per workgroup in LDS, the rest as layout 2)
+LAYOUTS = (1, 8, 2, 3, 4, 5, 6)
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -174,7 +175,7 @@ def test_layout_choice():
     ref = gk.apply(3, 2, c["table"], 4, 2, c["vA"], 1.0, c["x"], 0.0, c["y"])
     assert rel(host(y), ref) < TOL64
     with pytest.raises(asgard_amd.AkmError):
-        m.set_tuning(waves_per_row=6)
+        m.set_tuning(waves_per_row=7)
     big = grid_case(2, 9, 2, 1, seed=15)  # n^d = 81: no wavefront-per-cell layout
     mb = global_matrix(big)
     with pytest.raises(asgard_amd.AkmError):
@@ -220,3 +221,102 @@ def test_same_operator_as_pairwise_at_bench_size():
         y2 = dev(np.zeros_like(c["x"]))
         glob.apply(1.0, x, 0.0, y2)
         assert rel(host(y2), host(y1)) < 1e-12, layout
+
+
+def identity_for(d, active_sets):
+    """(term, dimension) pairs outside each term's active set."""
+    return tuple((t, k) for t, act in enumerate(active_sets) for k in range(d) if k not in act)
+
+
+# terms with one, two and three non-identity dimensions (the slab layout
+# groups the first two kinds by dimension set; three go to the cell layout)
+SLAB_CASES = [
+    dict(d=4, n=2, L=5, act=[{0}, {1, 2}, {0, 1, 2}, {1, 2}]),
+    dict(d=3, n=3, L=4, act=[{0, 2}, {1}, {0, 2}], refine=0.3),
+    dict(d=6, n=2, L=4, act=[{0, 3}, {1, 4}, {2, 5}, {0, 3}, {1, 4}, {2, 5}]),
+    dict(d=5, n=2, L=4, act=[{4}, {0, 4}, {1, 3}, {0, 1, 2, 3, 4}], refine=0.3),
+    dict(d=3, n=4, L=5, act=[{0, 1}, {2}]),
+    dict(d=2, n=4, L=4, act=[{0, 1}, {1}]),
+    dict(d=1, n=3, L=5, act=[{0}, {0}]),
+    # one slab of 576 cells x 4 values: the 1024-thread workgroups
+    dict(d=2, n=2, L=7, act=[{0, 1}, {0}]),
+    # a slab of 2816 cells x 4 values does not fit: that group runs as layout 2
+    dict(d=2, n=2, L=9, act=[{0, 1}, {1}]),
+]
+
+
+@pytest.mark.parametrize("spec", SLAB_CASES, ids=lambda s: f"d{s['d']}n{s['n']}L{s['L']}"
+                         + ("r" if s.get("refine") else ""))
+def test_slab_layout_parity(spec):
+    d, n, L = spec["d"], spec["n"], spec["L"]
+    T = len(spec["act"])
+    c = grid_case(d, n, L, T, seed=31, refine=spec.get("refine", 0.0),
+                  identity=identity_for(d, spec["act"]))
+    ref = gk.apply(d, n, c["table"], L, T, c["vA"], 1.5, c["x"], -0.5, c["y"],
+                   active=c["active"])
+    m = global_matrix(c, active=True)
+    m.set_tuning(waves_per_row=6)
+    y = dev(c["y"])
+    m.apply(1.5, dev(c["x"]), -0.5, y)
+    assert rel(host(y), ref) < TOL64
+    y2 = dev(c["y"])
+    m.apply(1.5, dev(c["x"]), -0.5, y2)
+    assert np.array_equal(host(y), host(y2))
+    # the cell layout on the same handle
+    m.set_tuning(waves_per_row=2 if n ** d <= 64 else 1)
+    y3 = dev(c["y"])
+    m.apply(1.5, dev(c["x"]), -0.5, y3)
+    assert rel(host(y), host(y3)) < TOL64
+    m32 = global_matrix(c, np.float32, active=True)
+    m32.set_tuning(waves_per_row=6)
+    y32 = dev(c["y"].astype(np.float32))
+    m32.apply(1.5, dev(c["x"].astype(np.float32)), -0.5, y32)
+    assert rel(host(y32), ref) < TOL32
+
+
+def test_slab_layout_flux_direction_and_solve():
+    from tests import global_dense
+    from tests.kron_cases import grid_case_ops
+
+    act = [{0, 2}, {1, 2}, {2}]
+    c, ops = grid_case_ops(3, 2, 4, 3, seed=33, refine=0.3, identity=identity_for(3, act))
+    flux = [1, 1, 0]  # terms 0 and 1 start with their second active dimension
+    m = global_matrix(c, active=True, flux=flux)
+    m.set_tuning(waves_per_row=6)
+    y = dev(np.zeros_like(c["x"]))
+    m.apply(1.0, dev(c["x"]), 0.0, y)
+    want = global_dense.apply(3, 2, 4, 3, ops, c["table"], c["x"], active=c["active"],
+                              flux_dim=flux)
+    assert rel(host(y), want) < TOL64
+    # the Jacobi-preconditioned implicit solve with the slab layout
+    dt = 0.01
+    b = c["y"]
+    info = asgard_amd.simple_gmres_euler(dt, m, dev(np.zeros_like(b)), dev(b), 20, 20, 1e-11)
+    m.set_tuning(waves_per_row=2)
+    info2 = asgard_amd.simple_gmres_euler(dt, m, dev(np.zeros_like(b)), dev(b), 20, 20, 1e-11)
+    assert abs(info.iterations - info2.iterations) <= 1 and info.error <= 1e-11
+
+
+def test_slab_layout_vlasov_d6_full_size():
+    # d=6 n=2 |l|_1 <= 7, the 3x+3v Vlasov operator: on a standard sparse grid
+    # the global operator equals the pair operator
+    from asgard_amd import dg
+
+    d, n, L, T = 6, 2, 7, 6
+    table, _ = oracle.element_table([L] * d)
+    ops = dg.operators_vlasov(n, L)
+    vA = oracle.build_vA(d, n, T, L, ops)
+    pair = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
+    glob = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA)
+    rows = len(oracle.cell_1d_index(table))
+    x = dev(np.random.default_rng(35).uniform(-1, 1, rows * n ** d))
+    y1 = dev(np.zeros(rows * n ** d))
+    pair.apply(1.0, x, 0.0, y1)
+    glob.set_tuning(waves_per_row=6)
+    y2 = dev(np.zeros(rows * n ** d))
+    glob.apply(1.0, x, 0.0, y2)
+    assert rel(host(y2), host(y1)) < 1e-12
+    glob.set_tuning(waves_per_row=3)
+    y3 = dev(np.zeros(rows * n ** d))
+    glob.apply(1.0, x, 0.0, y3)
+    assert rel(host(y2), host(y3)) < 1e-13
