@@ -45,6 +45,7 @@ EXPORTS = (
     "akm_last_error", "akm_version", "akm_device_count", "akm_set_stream",
     "akm_synchronize", "akm_connect_1d", "akm_sparse_pattern", "akm_create_sparse",
     "akm_create_sparse_grid", "akm_create_global_grid", "akm_global_info",
+    "akm_global_slab_info",
     "akm_global_jacobi", "akm_create_dense", "akm_update_coefficients",
     "akm_update_operators", "akm_imex_step", "akm_dg_term",
     "akm_apply", "akm_apply_euler", "akm_matrix_info", "akm_set_tuning",
@@ -130,6 +131,8 @@ def load_library(path: str = LIB_PATH):
                                          ctypes.POINTER(_vp)]
     L.akm_create_global_grid.argtypes = [_i, _i, _i64, _vp, _i, _i, _vp, _i64, _vp, _vp, _i,
                                          ctypes.POINTER(_vp)]
+    L.akm_global_slab_info.argtypes = [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                       ctypes.POINTER(_i)]
     L.akm_global_info.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i),
                                   ctypes.POINTER(_i)]
     L.akm_global_jacobi.argtypes = [_vp, _vp]
@@ -329,6 +332,15 @@ class KronmultMatrix:
         _check(load_library().akm_global_info(self._h, ctypes.byref(p), ctypes.byref(c),
                                               ctypes.byref(st)))
         return p.value, c.value, st.value
+
+    def slab_info(self):
+        """(term groups on the slab kernels, chains left to the cell-layout
+        stages, vectors the final sum reads) of a global operator's slab
+        layout."""
+        g, r, sc = _i(), _i(), _i()
+        _check(load_library().akm_global_slab_info(self._h, ctypes.byref(g), ctypes.byref(r),
+                                                   ctypes.byref(sc)))
+        return g.value, r.value, sc.value
 
     def jacobi(self):
         """The Jacobi diagonal of a global operator (FP64, active DOFs)."""

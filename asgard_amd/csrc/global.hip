@@ -530,6 +530,10 @@ struct gs_item
   int tab0, ntab;   // the group's factor tables in gs_args::tabs
   int pass0, npass; // the group's passes in gs_args::passes
   int slot;         // work-buffer slot of the group's sum
+  // a split item (one chain of a large slab): its chain of the bundle's nch
+  // and the bundle's first partial in the partial buffer (-1: not split);
+  // cell c's partial of chain k is partial part + c * nch + k (N values)
+  int part, nch, chain;
 };
 
 struct gs_args
@@ -563,7 +567,19 @@ __host__ __device__ inline int64_t gs_lds_bytes(int esize, int nd, int n, int nc
 template<int TPB, typename T>
 __device__ __forceinline__ void gs_copy(T *dst, const T *__restrict__ src, int count)
 {
-  for (int base = threadIdx.x; base < count; base += gs_unroll * TPB)
+  // whole batches without bounds tests, then the remainder
+  const int full = count / (gs_unroll * TPB) * (gs_unroll * TPB);
+  for (int base = threadIdx.x; base < full; base += gs_unroll * TPB)
+  {
+    T v[gs_unroll];
+#pragma unroll
+    for (int u = 0; u < gs_unroll; u++)
+      v[u] = src[base + u * TPB];
+#pragma unroll
+    for (int u = 0; u < gs_unroll; u++)
+      dst[base + u * TPB] = v[u];
+  }
+  for (int base = full + threadIdx.x; base < count; base += gs_unroll * TPB)
   {
     T v[gs_unroll];
 #pragma unroll
@@ -692,7 +708,7 @@ __device__ __forceinline__ void gs_pass_block(const P *in, const P *tab, const i
 template<typename P, int NT, int ND, int TPB>
 __global__ void __launch_bounds__(TPB)
     gk_slab(gs_args a, const P *__restrict__ ctab, const gs_item *__restrict__ items,
-            const P *__restrict__ x, P *work)
+            const P *__restrict__ x, P *work, P *part)
 {
   extern __shared__ __align__(16) unsigned char gs_smem[];
   constexpr int nn = NT * NT;
@@ -793,10 +809,13 @@ __global__ void __launch_bounds__(TPB)
   }
   if (active)
   {
-    P *out = work + int64_t(b.slot) * a.stride;
+    // the cell in the group's slot, or this chain's partial of the cell
+    // (split items); entries at their offsets within the cell
+    P *out = (b.part < 0) ? work + int64_t(b.slot) * a.stride + int64_t(gc) * a.N
+                          : part + (int64_t(b.part) + int64_t(c) * b.nch + b.chain) * a.N;
 #pragma unroll
     for (int si = 0; si < ds; si++)
-      out[gidx(si)] = acc[si];
+      out[gidx(si) - int64_t(gc) * a.N] = acc[si];
   }
 }
 
@@ -913,9 +932,9 @@ struct akm_global_s
   std::vector<gs_launch> gs_launches;
   // the 1024-thread launch (one large slab per workgroup, few workgroups)
   // runs beside the others on a stream of its own, forked and joined by events
-  int gs_nparts = 0; // partial slots of split items (after the group slots)
-  int gs_nred = 0;    // cells gs_reduce sums
-  akm::dev_buffer<int> gs_red; // (cell, group slot, chains) per reduced cell
+  int gs_nred = 0;                    // cells gs_reduce sums
+  akm::dev_buffer<int> gs_red;        // (cell, group slot, chains, first partial) each
+  akm::dev_buffer<unsigned char> gs_part; // split items' partials (N values each)
   hipStream_t gs_side = nullptr;
   hipEvent_t gs_fork = nullptr, gs_join = nullptr;
   ~akm_global_s()
@@ -974,7 +993,8 @@ static gk_args global_args(akm_matrix_s const &m)
 //! Launches one slab work-item class: `count` items from `items`.
 template<typename P, int TPB>
 static int gs_launch_class(hipStream_t stream, int n, int nd, gs_args const &ga, const P *ctab,
-                           const gs_item *items, int count, int64_t lds, const P *x, P *work)
+                           const gs_item *items, int count, int64_t lds, const P *x, P *work,
+                           P *part)
 {
   auto go = [&](auto kern) -> int {
     if (lds > 64 * 1024 &&
@@ -986,7 +1006,7 @@ static int gs_launch_class(hipStream_t stream, int n, int nd, gs_args const &ga,
       return set_error(AKM_ERR_HIP, "global kron slab: %lld bytes of LDS refused",
                        static_cast<long long>(lds));
     }
-    kern<<<count, TPB, static_cast<size_t>(lds), stream>>>(ga, ctab, items, x, work);
+    kern<<<count, TPB, static_cast<size_t>(lds), stream>>>(ga, ctab, items, x, work, part);
     return check_launch("global kron slab");
   };
   switch (n * 4 + nd)
@@ -1037,23 +1057,25 @@ __global__ void __launch_bounds__(gk_threads)
 }
 
 //! The group sums of split bundles: slot[e] = sum over the chains, in
-//! chain order, of their partial slots (pslot0 + k), for the N entries of
-//! each listed cell (red: cell, group slot, chains).
+//! chain order, of the cell's partials, for the N entries of each listed
+//! cell (red: cell, group slot, chains, first partial).
 template<typename P>
 __global__ void __launch_bounds__(gk_threads)
-    gs_reduce(const int *__restrict__ red, int nred, int N, int64_t stride, int pslot0, P *work)
+    gs_reduce(const int *__restrict__ red, int nred, int N, int64_t stride,
+              const P *__restrict__ part, P *work)
 {
   const int64_t total = int64_t(nred) * N;
   for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < total;
        i += int64_t(gk_threads) * gridDim.x)
   {
     const int64_t r = i / N;
-    const int64_t at = int64_t(red[3 * r]) * N + (i - r * N);
-    const int nch = red[3 * r + 2];
-    P sum = work[int64_t(pslot0) * stride + at];
+    const int64_t e = i - r * N;
+    const int nch = red[4 * r + 2];
+    const int64_t p0 = red[4 * r + 3];
+    P sum = part[p0 * N + e];
     for (int k = 1; k < nch; k++)
-      sum += work[int64_t(pslot0 + k) * stride + at];
-    work[int64_t(red[3 * r + 1]) * stride + at] = sum;
+      sum += part[(p0 + k) * N + e];
+    work[int64_t(red[4 * r + 1]) * stride + int64_t(red[4 * r]) * N + e] = sum;
   }
 }
 
@@ -1063,8 +1085,8 @@ static int gs_reduce_launch(akm_matrix_s &m, hipStream_t stream, P *work)
   akm_global_s &g = *m.gk;
   const int64_t total = int64_t(g.gs_nred) * m.N;
   gs_reduce<P><<<gk_blocks(total), gk_threads, 0, stream>>>(
-      g.gs_red.data(), g.gs_nred, static_cast<int>(m.N), g.cells * m.N, 2 * g.nslots + g.ngroups,
-      work);
+      g.gs_red.data(), g.gs_nred, static_cast<int>(m.N), g.cells * m.N,
+      reinterpret_cast<const P *>(g.gs_part.data()), work);
   return check_launch("global kron slab reduce");
 }
 
@@ -1109,10 +1131,13 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
     auto const &l = g.gs_launches[li];
     const hipStream_t s = l.side ? side : m.stream;
     const gs_item *items = reinterpret_cast<const gs_item *>(g.gs_items.data()) + l.first;
+    P *part = reinterpret_cast<P *>(g.gs_part.data());
     if (l.tpb == 1024)
-      AKM_TRY((gs_launch_class<P, 1024>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work)));
+      AKM_TRY((gs_launch_class<P, 1024>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work,
+                                        part)));
     else
-      AKM_TRY((gs_launch_class<P, 256>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work)));
+      AKM_TRY((gs_launch_class<P, 256>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work,
+                                       part)));
   }
   if (fork)
   {
@@ -1797,7 +1822,23 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     return cls(u) != cls(v) ? cls(u) < cls(v) : u.lds > v.lds;
   });
   g.gs_launches.clear();
-  int nparts = 0;
+  // partials of split bundles: per bundle ncell * chains of N values
+  const size_t nbundles = bundles.size() / (sizeof(gs_bundle) / sizeof(int));
+  std::vector<int64_t> part0(nbundles, -1);
+  std::vector<int> nch_of(nbundles, 0);
+  int64_t nparts = 0;
+  for (auto const &r : irec)
+    if (r.chain >= 0)
+      nch_of[r.bundle] = std::max(nch_of[r.bundle], r.chain + 1);
+  for (size_t bi = 0; bi < nbundles; bi++)
+    if (nch_of[bi] > 0)
+    {
+      part0[bi] = nparts;
+      nparts += int64_t(reinterpret_cast<const gs_bundle *>(bundles.data())[bi].ncell) *
+                nch_of[bi];
+    }
+  if (nparts > (int64_t(1) << 31) - 1)
+    return set_error(AKM_ERR_UNSUPPORTED, "global kron: too many slab partials");
   // compact factor tables, per level bound every group table: cbase[Lp]
   const int64_t ntab_all = static_cast<int64_t>(tabs.size() / 2);
   std::vector<int64_t> cbase(L + 2, 0);
@@ -1830,10 +1871,11 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     const bool split = irec[i].chain >= 0;
     it.pass0 = gg.pass0 + (split ? irec[i].pass_off : 0);
     it.npass = split ? irec[i].pass_cnt : gg.npass;
-    // a split item writes its chain's partial slot, summed by gs_reduce
-    it.slot = split ? 2 * g.nslots + ngroups + irec[i].chain : gg.slot;
-    if (split)
-      nparts = std::max(nparts, irec[i].chain + 1);
+    // a split item writes its chain's partials, summed by gs_reduce
+    it.slot = gg.slot;
+    it.part = split ? part0[irec[i].bundle] : -1;
+    it.nch = split ? nch_of[irec[i].bundle] : 0;
+    it.chain = split ? irec[i].chain : 0;
     const int *ip = reinterpret_cast<const int *>(&it);
     items.insert(items.end(), ip, ip + sizeof(gs_item) / sizeof(int));
     if (g.gs_launches.empty() || cls(irec[i - 1]) != cls(irec[i]))
@@ -1843,34 +1885,28 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     l.lds = std::max(l.lds, irec[i].lds);
     l.count++;
   }
-  // gs_reduce: the active cells of split bundles, their group slot and chain
-  // count (every chunk of a split bundle is split, so each cell once)
+  // gs_reduce: the active cells of split bundles (every chunk of a split
+  // bundle is split, so each cell once), their group slot, chain count and
+  // first partial
   std::vector<int> red;
+  for (size_t bi = 0; bi < nbundles; bi++)
   {
-    std::vector<bool> seen(bundles.size() / (sizeof(gs_bundle) / sizeof(int)), false);
-    for (auto const &r : irec)
+    if (nch_of[bi] == 0)
+      continue;
+    const gs_bundle &bb = hb[bi];
+    for (int c = 0; c < bb.ncell; c++)
     {
-      if (r.chain < 0 || seen[r.bundle])
-        continue;
-      seen[r.bundle] = true;
-      const gs_bundle &bb = hb[r.bundle];
-      const gs_group &gg = hg[bb.group];
-      int nch = 0;
-      for (auto const &r2 : irec)
-        if (r2.bundle == r.bundle && r2.h == r.h)
-          nch++;
-      for (int c = 0; c < bb.ncell; c++)
-      {
-        const int gc = cells[bb.cell0 + c];
-        if (gc < g.active_cells)
-          red.insert(red.end(), {gc, gg.slot, nch});
-      }
+      const int gc = cells[bb.cell0 + c];
+      if (gc < g.active_cells)
+        red.insert(red.end(), {gc, hg[bb.group].slot, nch_of[bi],
+                               static_cast<int>(part0[bi] + int64_t(c) * nch_of[bi])});
     }
   }
-  g.gs_nparts = nparts;
-  g.gs_nred = static_cast<int>(red.size() / 3);
+  g.gs_nred = static_cast<int>(red.size() / 4);
   if (!red.empty())
     AKM_TRY(g.gs_red.upload(red.data(), int64_t(red.size())));
+  if (nparts > 0)
+    AKM_TRY(g.gs_part.allocate(nparts * m.N * esize));
   if (!g.gs_launches.empty() && g.gs_launches[0].side && !g.gs_launches.back().side &&
       g.gs_side == nullptr)
   {
@@ -2177,7 +2213,7 @@ static int build_global(akm_matrix_s &m, const std::vector<int> &elem_active,
     AKM_TRY(g->first_chain.upload(gchain.data(), int64_t(gchain.size())));
   }
   AKM_TRY(build_slabs(m, *g, lists, term_act, ch_term, ch_nst, ch_dim, ch_fill, ch_slot));
-  const int64_t work_slots = int64_t(2) * nslots + g->ngroups + g->gs_nparts;
+  const int64_t work_slots = int64_t(2) * nslots + g->ngroups;
   if (work_slots > 0)
     AKM_TRY(g->work.allocate(work_slots * g->cells * m.N * int64_t(m.esize)));
   // flops as the reference counts them (src/asgard_kronmult_matrix.cpp:
@@ -2268,6 +2304,20 @@ akm_status akm_create_global_grid(int d, int n, int64_t num_cells, const int *ta
   if (T > 0)
     AKM_TRY(global_update(*m, vA));
   *mat = m.release();
+  return AKM_SUCCESS;
+}
+
+akm_status akm_global_slab_info(akm_matrix mat, int *groups, int *rest_chains, int *sum_sources)
+{
+  if (mat == nullptr || mat->mode != mode_global || !mat->gk)
+    return set_error(AKM_ERR_INVALID, "global slab info: not a global kron matrix");
+  akm_global_s const &g = *mat->gk;
+  if (groups)
+    *groups = g.ngroups;
+  if (rest_chains)
+    *rest_chains = g.max_stages > 0 ? g.stage_count_rest[0] : 0;
+  if (sum_sources)
+    *sum_sources = g.nsrc_slab;
   return AKM_SUCCESS;
 }
 

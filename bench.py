@@ -464,8 +464,10 @@ def global_operator_applies(reps=20):
     """Applies of the global Kronecker operator (the reference's default
     mode) on the two solve workloads' operators and on the bench workload's
     random factors; per apply the time, the layout the autotuner chose and
-    the algorithmic bytes (every stage reads and writes one padded vector
-    per chain, the chain sum reads one vector per chain and writes y)."""
+    the algorithmic bytes (chain layouts: every stage reads and writes one
+    padded vector per chain, the chain sum reads one vector per chain and
+    writes y; slab layout: every term group reads x and writes one vector,
+    the sum reads the group vectors and writes y)."""
     import torch
 
     import asgard_amd
@@ -499,7 +501,15 @@ def global_operator_applies(reps=20):
             dofs = m.output_size()
             padded = dofs + pad * n ** d
             esize = np.dtype(dtype).itemsize
-            moved = (chains * stages * 2 * padded + (chains + 1) * dofs) * esize
+            if layout == 6:
+                # slab layout: every group reads x and writes its sum once, the
+                # remaining chains run stage by stage, the final sum reads its
+                # sources and writes y
+                groups, rest, sources = m.slab_info()
+                moved = (2 * groups * dofs + rest * stages * 2 * padded +
+                         (sources + 1) * dofs) * esize
+            else:
+                moved = (chains * stages * 2 * padded + (chains + 1) * dofs) * esize
             rec[prec] = {"us_per_apply": round(ms * 1e3, 2), "layout": layout,
                          "chains": chains, "stages": stages,
                          "algorithmic_bytes": int(moved),

@@ -134,6 +134,14 @@ akm_status akm_create_global_grid(int d, int n, int64_t num_cells, const int *ta
  * reference (its constructor prints the padding count). */
 akm_status akm_global_info(akm_matrix mat, int64_t *padded_cells, int *chains, int *stages);
 
+/* The slab layout of a global operator (akm_set_tuning layout 6): term
+ * groups handled by the slab kernels (terms with the same one or two
+ * non-identity dimensions), chains left to the cell-layout stages, and the
+ * vectors the final sum reads (group sums, remaining chains' results, x for
+ * identity terms).  No counterpart in the reference. */
+akm_status akm_global_slab_info(akm_matrix mat, int *groups, int *rest_chains,
+                                int *sum_sources);
+
 /* The Jacobi diagonal of a global operator (build_preconditioner,
  * src/asgard_kronmult_matrix.cpp:1016-1067), one FP64 value per active DOF,
  * copied to the host array pc. */

@@ -234,13 +234,16 @@ SLAB_CASES = [
     dict(d=4, n=2, L=5, act=[{0}, {1, 2}, {0, 1, 2}, {1, 2}]),
     dict(d=3, n=3, L=4, act=[{0, 2}, {1}, {0, 2}], refine=0.3),
     dict(d=6, n=2, L=4, act=[{0, 3}, {1, 4}, {2, 5}, {0, 3}, {1, 4}, {2, 5}]),
+    # level-5 slabs run one chain per workgroup; the cells near the origin lie
+    # in such a slab of every group, each group's chain partials kept apart
+    dict(d=6, n=2, L=5, act=[{0, 3}, {1, 4}, {2, 5}, {0, 3}, {1, 4}, {2, 5}]),
     dict(d=5, n=2, L=4, act=[{4}, {0, 4}, {1, 3}, {0, 1, 2, 3, 4}], refine=0.3),
     dict(d=3, n=4, L=5, act=[{0, 1}, {2}]),
     dict(d=2, n=4, L=4, act=[{0, 1}, {1}]),
     dict(d=1, n=3, L=5, act=[{0}, {0}]),
     # one slab of 576 cells x 4 values: the 1024-thread workgroups
     dict(d=2, n=2, L=7, act=[{0, 1}, {0}]),
-    # a slab of 2816 cells x 4 values does not fit: that group runs as layout 2
+    # level-9 factor tables do not fit LDS: every term runs as layout 2
     dict(d=2, n=2, L=9, act=[{0, 1}, {1}]),
 ]
 
@@ -256,6 +259,14 @@ def test_slab_layout_parity(spec):
                    active=c["active"])
     m = global_matrix(c, active=True)
     m.set_tuning(waves_per_row=6)
+    groups, rest, sources = m.slab_info()
+    sets = {frozenset(a) for a in spec["act"] if 1 <= len(a) <= 2}
+    if spec["L"] == 9:
+        # the level-9 factor tables (about 12,000 blocks) exceed LDS: both
+        # groups fall back to the cell layout
+        assert groups == 0 and rest == 3
+    else:
+        assert groups == len(sets)
     y = dev(c["y"])
     m.apply(1.5, dev(c["x"]), -0.5, y)
     assert rel(host(y), ref) < TOL64
@@ -300,19 +311,23 @@ def test_slab_layout_flux_direction_and_solve():
 def test_slab_layout_vlasov_d6_full_size():
     # d=6 n=2 |l|_1 <= 7, the 3x+3v Vlasov operator: on a standard sparse grid
     # the global operator equals the pair operator
-    from asgard_amd import dg
+    from asgard_amd import dg, grid
 
     d, n, L, T = 6, 2, 7, 6
     table, _ = oracle.element_table([L] * d)
     ops = dg.operators_vlasov(n, L)
     vA = oracle.build_vA(d, n, T, L, ops)
     pair = asgard_amd.KronmultMatrix.sparse_grid(d, n, table, L, T, vA)
-    glob = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA)
+    # identity factors skipped, as the reference does for its mass terms
+    act = grid.identity_mask(ops, T, d)
+    glob = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA, active=act)
     rows = len(oracle.cell_1d_index(table))
     x = dev(np.random.default_rng(35).uniform(-1, 1, rows * n ** d))
     y1 = dev(np.zeros(rows * n ** d))
     pair.apply(1.0, x, 0.0, y1)
     glob.set_tuning(waves_per_row=6)
+    # three groups (x_i, v_i) of two terms; the sum reads the three group sums
+    assert glob.slab_info() == (3, 0, 3)
     y2 = dev(np.zeros(rows * n ** d))
     glob.apply(1.0, x, 0.0, y2)
     assert rel(host(y2), host(y1)) < 1e-12
