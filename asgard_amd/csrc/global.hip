@@ -1567,6 +1567,10 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
           key.push_back(g.elem_host[c * d + k]);
       slabs[key].push_back(static_cast<int>(c));
     }
+    int group_chains = 0;
+    for (size_t c = 0; c < ch_term.size(); c++)
+      if (term_group[ch_term[c]] == static_cast<int>(gi))
+        group_chains++;
     // bundles: small slabs packed up to gs_cap_small values, a larger slab
     // alone with its other-index combinations split into chunks
     std::vector<gs_plan_bundle> plan;
@@ -1600,7 +1604,9 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
         open_vals += full;
         continue;
       }
-      const int64_t cap = (msz * ds <= gs_cap_small) ? gs_cap_small : gs_cap_large;
+      // a slab whose chains run split (long lists) takes the widest chunks
+      const bool wide = lp >= gs_split_level && group_chains > 1;
+      const int64_t cap = (msz * ds <= gs_cap_small && !wide) ? gs_cap_small : gs_cap_large;
       if (msz * ds > cap)
       {
         ok = false;
