@@ -1619,6 +1619,10 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
       one.C = call;
       while (msz * ds * one.C > cap)
         one.C /= n;
+      // x and the intermediate of a wide chunk within 64 KB, which leaves LDS
+      // for the factor table and the lists
+      while (wide && one.C > 1 && msz * ds * one.C * esize * nd > 64 * 1024)
+        one.C /= n;
       plan.push_back(one);
     }
     if (ok && !open.cells.empty())
