@@ -89,6 +89,40 @@ def test_imex_step_matches_oracle(case, collision):
         assert rel(got, want) < 1e-13
 
 
+@pytest.mark.parametrize("d,n,lev,terms", [(3, 2, 6, 3), (4, 3, 4, 2)])
+def test_imex_step_larger_grids(d, n, lev, terms):
+    # the collision solve on grids of 5,504 and 15,552 unknowns; the implicit
+    # factors are scaled by 1/sqrt(size) so that dt*A stays well conditioned
+    ce = sparse_grid_case(d, n, lev, terms, seed=41)
+    rng = np.random.default_rng(42)
+    s = n << lev
+    ops_b = [np.asfortranarray(rng.uniform(-1, 1, (s, s)) / np.sqrt(s))
+             for _ in range(terms * d)]
+    vA_b = oracle.build_vA(d, n, terms, lev, ops_b)
+    E = asgard_amd.KronmultMatrix.sparse_grid(d, n, ce["table"], lev, terms, ce["vA"])
+    B = asgard_amd.KronmultMatrix.sparse_grid(d, n, ce["table"], lev, terms, vA_b)
+    f = torch.from_numpy(ce["x"].copy()).cuda()
+    infos, converged = asgard_amd.imex_step(E, B, DT, f, RESTART, MAX_OUTER, TOL,
+                                            collision=True)
+    pntr, indx, iA = ce["pntr"], ce["indx"], ce["iA"]
+
+    def explicit_apply(x):
+        y = np.zeros_like(x)
+        oracle.kronmult_sparse(d, n, pntr, indx, terms, iA, ce["vA"], 1.0, x, 0.0, y)
+        return y
+
+    def implicit_solve(c, guess, rhs):
+        x = np.array(guess, dtype=np.float64)
+        oracle.gmres_euler_sparse(d, n, pntr, indx, terms, iA, vA_b, c, x, rhs, RESTART,
+                                  MAX_OUTER, TOL)
+        return x
+
+    want = imex.imex_step(explicit_apply, implicit_solve, DT, ce["x"], True, None)
+    assert converged
+    assert all(i.iterations > 0 and i.error <= TOL for i in infos)
+    assert rel(f.cpu().numpy(), want) < 1e-9
+
+
 def test_imex_hook_updates_operator(case):
     ce, ops_b, vA_b = case
     E, B = handles(ce, vA_b)

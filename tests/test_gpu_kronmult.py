@@ -268,6 +268,23 @@ def test_dense_mode(d, n):
     assert rel(host(y), ref) < TOL64
 
 
+@pytest.mark.parametrize("d,n,L", [(3, 2, 5), (2, 3, 6), (4, 2, 4)])
+def test_dense_mode_larger_grids(d, n, L):
+    # every cell pair of the element table coupled (the reference's dense mode)
+    T = 2
+    c = sparse_grid_case(d, n, L, T, seed=10)
+    nb = 1 << L
+    terms = oracle.build_dense_terms(d, n, T, nb, c["ops"])
+    cells = c["rows"]
+    ref = c["y"].copy()
+    oracle.kronmult_dense(d, n, cells, cells, T, c["elem"], 0, 0, terms, nb, 1.25, c["x"], 0.5,
+                          ref)
+    m = asgard_amd.KronmultMatrix.dense(d, n, cells, cells, T, terms, c["elem"], 0, 0, nb)
+    y = dev(c["y"])
+    m.apply(1.25, dev(c["x"]), 0.5, y)
+    assert rel(host(y), ref) < TOL64
+
+
 def test_host_pointer_apply_and_update():
     c = sparse_grid_case(2, 3, 4, 2, seed=6)
     m = asgard_amd.KronmultMatrix.sparse_grid(2, 3, c["table"], 4, 2, c["vA"])
