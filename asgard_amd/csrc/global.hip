@@ -819,25 +819,57 @@ __global__ void __launch_bounds__(TPB)
   }
 }
 
+//! Where gk_sum finds the slab groups' sums of cells whose chains ran split:
+//! psplit[g * active cells + cell] is the cell's first partial in part (-1:
+//! the group's slot holds the sum), gnch[g] the group's chains; group g's
+//! slot is gslot0 + g.
+template<typename P>
+struct gs_split_view
+{
+  const int *psplit = nullptr;
+  const int *gnch = nullptr;
+  const P *part = nullptr;
+  int gslot0 = 0, ngroups = 0;
+};
+
 //! y = (alpha * sum of the sources + beta*y + gamma*xg) / (1 - jdt * pc) (no
 //! division when pc is null).  The sources, in order: x (src < 0, a term of
 //! identity factors) or work-buffer slot src (a chain's last stage in chain
-//! order, or a slab group's sum at the place of the group's first chain).
+//! order, or a slab group's sum at the place of the group's first chain; for
+//! a cell whose group chains ran split, the sum of its chain partials in
+//! chain order).
 template<typename P>
 __global__ void __launch_bounds__(gk_threads)
     gk_sum(gk_args a, const int *__restrict__ src, int nsrc, const P *__restrict__ x,
            const P *__restrict__ work, P alpha, P beta, P *y, P gamma, const P *xg,
-           const double *pc, double jdt)
+           const double *pc, double jdt, gs_split_view<P> sv)
 {
   const int64_t dofs = a.active_cells * a.N;
   for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < dofs;
        i += int64_t(gk_threads) * gridDim.x)
   {
     P sum = 0;
+    const int64_t cell = (sv.psplit != nullptr) ? i / a.N : 0;
+    const int64_t e = i - cell * a.N;
     for (int c = 0; c < nsrc; c++)
     {
       const int sl = src[c];
-      sum += (sl < 0) ? x[i] : work[int64_t(sl) * a.stride + i];
+      const int gi = sl - sv.gslot0;
+      const int pb = (sv.psplit != nullptr && gi >= 0 && gi < sv.ngroups)
+                         ? sv.psplit[int64_t(gi) * a.active_cells + cell]
+                         : -1;
+      P v;
+      if (sl < 0)
+        v = x[i];
+      else if (pb < 0)
+        v = work[int64_t(sl) * a.stride + i];
+      else
+      {
+        v = sv.part[int64_t(pb) * a.N + e];
+        for (int k = 1; k < sv.gnch[gi]; k++)
+          v += sv.part[(int64_t(pb) + k) * a.N + e];
+      }
+      sum += v;
     }
     P out = alpha * sum;
     if (beta != P{0})
@@ -932,8 +964,8 @@ struct akm_global_s
   std::vector<gs_launch> gs_launches;
   // the 1024-thread launch (one large slab per workgroup, few workgroups)
   // runs beside the others on a stream of its own, forked and joined by events
-  int gs_nred = 0;                    // cells gs_reduce sums
-  akm::dev_buffer<int> gs_red;        // (cell, group slot, chains, first partial) each
+  akm::dev_buffer<int> gs_psplit;     // per group and active cell: first partial or -1
+  akm::dev_buffer<int> gs_gnch;       // per group: its chains
   akm::dev_buffer<unsigned char> gs_part; // split items' partials (N values each)
   hipStream_t gs_side = nullptr;
   hipEvent_t gs_fork = nullptr, gs_join = nullptr;
@@ -1056,40 +1088,6 @@ __global__ void __launch_bounds__(gk_threads)
   }
 }
 
-//! The group sums of split bundles: slot[e] = sum over the chains, in
-//! chain order, of the cell's partials, for the N entries of each listed
-//! cell (red: cell, group slot, chains, first partial).
-template<typename P>
-__global__ void __launch_bounds__(gk_threads)
-    gs_reduce(const int *__restrict__ red, int nred, int N, int64_t stride,
-              const P *__restrict__ part, P *work)
-{
-  const int64_t total = int64_t(nred) * N;
-  for (int64_t i = blockIdx.x * int64_t(gk_threads) + threadIdx.x; i < total;
-       i += int64_t(gk_threads) * gridDim.x)
-  {
-    const int64_t r = i / N;
-    const int64_t e = i - r * N;
-    const int nch = red[4 * r + 2];
-    const int64_t p0 = red[4 * r + 3];
-    P sum = part[p0 * N + e];
-    for (int k = 1; k < nch; k++)
-      sum += part[(p0 + k) * N + e];
-    work[int64_t(red[4 * r + 1]) * stride + int64_t(red[4 * r]) * N + e] = sum;
-  }
-}
-
-template<typename P>
-static int gs_reduce_launch(akm_matrix_s &m, hipStream_t stream, P *work)
-{
-  akm_global_s &g = *m.gk;
-  const int64_t total = int64_t(g.gs_nred) * m.N;
-  gs_reduce<P><<<gk_blocks(total), gk_threads, 0, stream>>>(
-      g.gs_red.data(), g.gs_nred, static_cast<int>(m.N), g.cells * m.N,
-      reinterpret_cast<const P *>(g.gs_part.data()), work);
-  return check_launch("global kron slab reduce");
-}
-
 //! Every slab group's sum into its work-buffer slot.
 template<typename P>
 static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
@@ -1117,7 +1115,7 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
         reinterpret_cast<P *>(g.gs_ctab.data()));
     AKM_TRY(check_launch("global kron slab tables"));
   }
-  // the side launches come first and run on the side stream; gs_reduce
+  // the side launches come first and run on the side stream; gk_sum
   // follows the join
   const bool fork = g.gs_side != nullptr;
   if (fork)
@@ -1144,8 +1142,6 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
     AKM_HIP(hipEventRecord(g.gs_join, g.gs_side));
     AKM_HIP(hipStreamWaitEvent(m.stream, g.gs_join, 0));
   }
-  if (g.gs_nred > 0)
-    AKM_TRY(gs_reduce_launch<P>(m, m.stream, work));
   return AKM_SUCCESS;
 }
 
@@ -1225,8 +1221,17 @@ static int global_apply_typed(akm_matrix_s &m, P alpha, const P *x, P beta, P *y
     AKM_TRY(gs_apply<P>(m, vA, x, work));
   const int *src = slab ? g.src_slab.data() : g.src_chain.data();
   const int nsrc = slab ? g.nsrc_slab : g.nsrc_chain;
+  gs_split_view<P> sv{};
+  if (slab && g.gs_psplit.size() > 0)
+  {
+    sv.psplit = g.gs_psplit.data();
+    sv.gnch = g.gs_gnch.data();
+    sv.part = reinterpret_cast<const P *>(g.gs_part.data());
+    sv.gslot0 = 2 * g.nslots;
+    sv.ngroups = g.ngroups;
+  }
   gk_sum<P><<<gk_blocks(dofs), gk_threads, 0, m.stream>>>(a, src, nsrc, x, work, alpha, beta, y,
-                                                          gamma, xg, pc, jdt);
+                                                          gamma, xg, pc, jdt, sv);
   return check_launch("global kron sum");
 }
 
@@ -1881,7 +1886,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     const bool split = irec[i].chain >= 0;
     it.pass0 = gg.pass0 + (split ? irec[i].pass_off : 0);
     it.npass = split ? irec[i].pass_cnt : gg.npass;
-    // a split item writes its chain's partials, summed by gs_reduce
+    // a split item writes its chain's partials, which gk_sum adds
     it.slot = gg.slot;
     it.part = split ? part0[irec[i].bundle] : -1;
     it.nch = split ? nch_of[irec[i].bundle] : 0;
@@ -1895,26 +1900,28 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     l.lds = std::max(l.lds, irec[i].lds);
     l.count++;
   }
-  // gs_reduce: the active cells of split bundles (every chunk of a split
-  // bundle is split, so each cell once), their group slot, chain count and
-  // first partial
-  std::vector<int> red;
-  for (size_t bi = 0; bi < nbundles; bi++)
+  // gk_sum's view of split bundles (every chunk of a split bundle is split):
+  // per group and active cell the first of its chain partials
+  if (nparts > 0)
   {
-    if (nch_of[bi] == 0)
-      continue;
-    const gs_bundle &bb = hb[bi];
-    for (int c = 0; c < bb.ncell; c++)
+    std::vector<int> psplit(size_t(ngroups) * g.active_cells, -1), gnch(ngroups, 0);
+    for (size_t bi = 0; bi < nbundles; bi++)
     {
-      const int gc = cells[bb.cell0 + c];
-      if (gc < g.active_cells)
-        red.insert(red.end(), {gc, hg[bb.group].slot, nch_of[bi],
-                               static_cast<int>(part0[bi] + int64_t(c) * nch_of[bi])});
+      if (nch_of[bi] == 0)
+        continue;
+      const gs_bundle &bb = hb[bi];
+      gnch[bb.group] = nch_of[bi];
+      for (int c = 0; c < bb.ncell; c++)
+      {
+        const int gc = cells[bb.cell0 + c];
+        if (gc < g.active_cells)
+          psplit[size_t(bb.group) * g.active_cells + gc] =
+              static_cast<int>(part0[bi] + int64_t(c) * nch_of[bi]);
+      }
     }
+    AKM_TRY(g.gs_psplit.upload(psplit.data(), int64_t(psplit.size())));
+    AKM_TRY(g.gs_gnch.upload(gnch.data(), int64_t(gnch.size())));
   }
-  g.gs_nred = static_cast<int>(red.size() / 4);
-  if (!red.empty())
-    AKM_TRY(g.gs_red.upload(red.data(), int64_t(red.size())));
   if (nparts > 0)
     AKM_TRY(g.gs_part.allocate(nparts * m.N * esize));
   if (!g.gs_launches.empty() && g.gs_launches[0].side && !g.gs_launches.back().side &&
