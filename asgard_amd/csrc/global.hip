@@ -474,6 +474,7 @@ __global__ void __launch_bounds__(gk_threads)
 constexpr int gs_blocks_small = 256;
 constexpr int gs_blocks_large = 1024;
 constexpr int gs_lds_max = 160 * 1024;
+constexpr int gs_pad = 2; // values of padding after each cell's blocks in LDS
 constexpr int gs_split_level = 5; // slabs with level bound >= this run one chain per workgroup
 constexpr int gs_side_level = 6;  // split items of slabs from this level bound on: side stream
 constexpr int gs_lds_resident = 64 * 1024; // stage everything at once up to this size
@@ -557,7 +558,7 @@ __host__ __device__ inline int64_t gs_lds_bytes(int esize, int nd, int n, int nc
                                                 int ntab, int64_t nlist_ints, int npass)
 {
   const int64_t ds = (nd == 1) ? n : int64_t(n) * n;
-  const int64_t bd = int64_t(ncell) * ds * C;
+  const int64_t bd = int64_t(ncell) * (ds * C + gs_pad);
   const int64_t vals = bd * nd + int64_t(ntab) * nq * n * n;
   return int64_t(16) * npass + (vals * esize + 15) / 16 * 16 + 4 * nlist_ints;
 }
@@ -715,7 +716,9 @@ __global__ void __launch_bounds__(TPB)
   constexpr int ds = (ND == 1) ? NT : nn;
   const gs_item b = items[blockIdx.x];
   const int C = b.C;
-  const int D = ds * C;
+  // a cell's blocks in LDS, padded by gs_pad values so that the blocks of
+  // neighbouring cells fall on different LDS banks
+  const int D = ds * C + gs_pad;
   const int BD = b.ncell * D;
   const int nq = b.nq;
   const int ntab_lds = b.resident ? b.ntab : 1;
@@ -1652,6 +1655,21 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
         npass += ch_nst[c];
       }
 
+    // each bundle's cells by their longest list, longest first: a wavefront
+    // walks as many connections as its longest list, so long lists share
+    // wavefronts (the order changes no operation order within a value)
+    for (auto &pb : plan)
+    {
+      auto longest = [&](int gc) {
+        int len = 0;
+        for (int k : S)
+          len = std::max(len, lists[k].ptr[gc + 1] - lists[k].ptr[gc]);
+        return len;
+      };
+      std::stable_sort(pb.cells.begin(), pb.cells.end(),
+                       [&](int u, int v) { return longest(u) > longest(v); });
+    }
+
     // lists, LDS sizes and items of the plan (checked before anything is kept)
     const int group_index = ngroups;
     std::vector<int> b_rec, p_cells, p_lists;
@@ -1687,7 +1705,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
           {
             // element offset of the connected cell in x / the intermediate,
             // element offset of its factor block in the table
-            const int64_t xo = int64_t(loc[ls.cell[p]]) * ds * pb.C;
+            const int64_t xo = int64_t(loc[ls.cell[p]]) * (ds * pb.C + gs_pad);
             const int64_t ao = int64_t(qc_of[pb.Lp][ls.q[p]]) * nn;
             if (xo < 0 || ao < 0 || xo > 0xffff || ao > 0xffff)
             {
