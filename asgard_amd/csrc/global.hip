@@ -476,7 +476,6 @@ constexpr int gs_blocks_large = 1024;
 constexpr int gs_lds_max = 160 * 1024;
 constexpr int gs_pad = 2; // values of padding after each cell's blocks in LDS
 constexpr int gs_split_level = 5; // slabs with level bound >= this run one chain per workgroup
-constexpr int gs_side_level = 6;  // split items of slabs from this level bound on: side stream
 constexpr int gs_lds_resident = 64 * 1024; // stage everything at once up to this size
 constexpr int gs_unroll = 8;               // independent loads per thread while staging
 
@@ -949,12 +948,15 @@ struct akm_global_s
   bool slab = false; // chosen by global_autotune or akm_set_tuning
   int ngroups = 0;
   akm::dev_buffer<int> gs_passes, gs_tabs, gs_items, gs_cells, gs_lists, gs_offs, gs_qmap;
-  // compact factor tables (per level bound, every group table), rebuilt from
-  // vA at each slab apply by gs_tables; gs_lv: qmap starts then table starts
-  // per level bound (gs_levels + 1 each)
+  // compact factor tables (per level bound, every group table), built from
+  // vA by gs_tables at the first slab apply after vA changed (the matrix's
+  // vA_stamp differs from gs_ctab_stamp); gs_lv: qmap starts then table
+  // starts per level bound (gs_levels + 1 each)
   akm::dev_buffer<unsigned char> gs_ctab;
   akm::dev_buffer<int> gs_lv;
   int64_t gs_ctab_count = 0;
+  uint64_t gs_ctab_stamp = 0;
+  bool gs_ctab_valid = false;
   int gs_levels = 0;
   struct gs_launch
   {
@@ -965,6 +967,12 @@ struct akm_global_s
     int first, count; // its items
   };
   std::vector<gs_launch> gs_launches;
+  std::vector<int64_t> gs_item_lds; // dynamic LDS bytes per item
+  // the last gs_moved items of the side launches run on the main stream after
+  // its own launches (chosen by global_autotune: the stream that ends last
+  // should be the main one, whose join then costs no wait)
+  int gs_moved = 0;
+  int gs_side_items = 0;
   // the 1024-thread launch (one large slab per workgroup, few workgroups)
   // runs beside the others on a stream of its own, forked and joined by events
   akm::dev_buffer<int> gs_psplit;     // per group and active cell: first partial or -1
@@ -1107,8 +1115,9 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
   ga.cells = g.gs_cells.data();
   ga.lists = g.gs_lists.data();
   ga.offs = g.gs_offs.data();
-  // the compact factor tables from the current vA
+  // the compact factor tables, rebuilt when vA changed since they were built
   const P *ctab = reinterpret_cast<const P *>(g.gs_ctab.data());
+  if (!g.gs_ctab_valid || g.gs_ctab_stamp != m.vA_stamp)
   {
     const int nn = m.n * m.n;
     const int64_t tstride = int64_t(m.d) * nn;
@@ -1117,6 +1126,8 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
         g.gs_lv.data(), g.gs_levels, nn, tstride, tstride * m.T, g.gs_ctab_count,
         reinterpret_cast<P *>(g.gs_ctab.data()));
     AKM_TRY(check_launch("global kron slab tables"));
+    g.gs_ctab_stamp = m.vA_stamp;
+    g.gs_ctab_valid = true;
   }
   // the side launches come first and run on the side stream; gk_sum
   // follows the join
@@ -1127,19 +1138,33 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
     AKM_HIP(hipStreamWaitEvent(g.gs_side, g.gs_fork, 0));
   }
   const hipStream_t side = fork ? g.gs_side : m.stream;
-  for (size_t li = 0; li < g.gs_launches.size(); li++)
-  {
-    auto const &l = g.gs_launches[li];
-    const hipStream_t s = l.side ? side : m.stream;
-    const gs_item *items = reinterpret_cast<const gs_item *>(g.gs_items.data()) + l.first;
-    P *part = reinterpret_cast<P *>(g.gs_part.data());
-    if (l.tpb == 1024)
-      AKM_TRY((gs_launch_class<P, 1024>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work,
-                                        part)));
-    else
-      AKM_TRY((gs_launch_class<P, 256>(s, m.n, l.nd, ga, ctab, items, l.count, l.lds, x, work,
-                                       part)));
-  }
+  // items [0, bound) of the side launches run on the side stream, the rest
+  // after the main stream's own launches
+  const int bound = fork ? g.gs_side_items - std::min(g.gs_moved, g.gs_side_items)
+                         : g.gs_side_items;
+  P *part = reinterpret_cast<P *>(g.gs_part.data());
+  auto run = [&](hipStream_t s, int tpb, int nd, int first, int count) -> int {
+    if (count <= 0)
+      return AKM_SUCCESS;
+    const gs_item *items = reinterpret_cast<const gs_item *>(g.gs_items.data()) + first;
+    const int64_t lds = *std::max_element(g.gs_item_lds.begin() + first,
+                                          g.gs_item_lds.begin() + first + count);
+    if (tpb == 1024)
+      return gs_launch_class<P, 1024>(s, m.n, nd, ga, ctab, items, count, lds, x, work, part);
+    return gs_launch_class<P, 256>(s, m.n, nd, ga, ctab, items, count, lds, x, work, part);
+  };
+  for (auto const &l : g.gs_launches)
+    if (l.side)
+      AKM_TRY(run(side, l.tpb, l.nd, l.first, std::min(l.first + l.count, bound) - l.first));
+  for (auto const &l : g.gs_launches)
+    if (!l.side)
+      AKM_TRY(run(m.stream, l.tpb, l.nd, l.first, l.count));
+  for (auto const &l : g.gs_launches)
+    if (l.side)
+    {
+      const int f = std::max(l.first, bound);
+      AKM_TRY(run(m.stream, l.tpb, l.nd, f, l.first + l.count - f));
+    }
   if (fork)
   {
     AKM_HIP(hipEventRecord(g.gs_join, g.gs_side));
@@ -1399,10 +1424,12 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
   bool best_grouped = false;
   int rc = AKM_SUCCESS;
   bool best_xcd = false, best_slab = false;
+  int best_moved = 0;
   // layouts 1 and 8, then the cell layout with and without the grouped
   // first stage (shared input reads against fewer, longer wavefronts), each
   // cell layout with and without the XCD-contiguous workgroup order, then
   // the slab layout (when some term has one or two non-identity dimensions)
+  // with 0-80 % of its side-stream items moved to the main stream
   const int shapes[7][4] = {{1, 0, 0, 0}, {8, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 1, 0},
                             {0, 1, 0, 0}, {0, 1, 1, 0}, {0, 0, 0, 1}};
   for (auto const &shape : shapes)
@@ -1421,28 +1448,38 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
     g.first_grouped = grouped;
     g.xcd_order = xcd;
     g.slab = slab;
-    rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
+    const int percents[8] = {0, 10, 20, 30, 40, 50, 65, 80};
+    const int nmoves = (slab && g.gs_side != nullptr) ? 8 : 1;
+    for (int mi = 0; mi < nmoves && rc == AKM_SUCCESS; mi++)
+    {
+      g.gs_moved = g.gs_side_items * percents[mi] / 100;
+      rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
+      if (rc != AKM_SUCCESS)
+        break;
+      (void)hipEventRecord(e0, m.stream);
+      for (int r = 0; r < 5 && rc == AKM_SUCCESS; r++)
+        rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
+      (void)hipEventRecord(e1, m.stream);
+      float ms = 0;
+      if (rc == AKM_SUCCESS && hipEventSynchronize(e1) == hipSuccess &&
+          hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (best < 0 || ms < best))
+      {
+        best = ms;
+        best_lanes = lanes;
+        best_grouped = grouped;
+        best_xcd = xcd;
+        best_slab = slab;
+        best_moved = g.gs_moved;
+      }
+    }
     if (rc != AKM_SUCCESS)
       break;
-    (void)hipEventRecord(e0, m.stream);
-    for (int r = 0; r < 5 && rc == AKM_SUCCESS; r++)
-      rc = global_apply(m, 1.0, x, 0.0, y, 0.0, nullptr, false, 0.0);
-    (void)hipEventRecord(e1, m.stream);
-    float ms = 0;
-    if (rc == AKM_SUCCESS && hipEventSynchronize(e1) == hipSuccess &&
-        hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (best < 0 || ms < best))
-    {
-      best = ms;
-      best_lanes = lanes;
-      best_grouped = grouped;
-      best_xcd = xcd;
-      best_slab = slab;
-    }
   }
   g.lanes = best_lanes;
   g.first_grouped = best_grouped;
   g.xcd_order = best_xcd;
   g.slab = best_slab;
+  g.gs_moved = best_moved;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
@@ -1838,12 +1875,11 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
 
   // launches: 1024-thread items, then 256-thread items above and below 32 KB
   // of LDS (the small ones keep more workgroups per CU)
-  // the side stream takes the 1024-thread items and the split items of the
-  // largest slabs (level bound >= gs_side_level), about half the work on
-  // the Vlasov operator; they come first
-  auto on_side = [](item_rec const &r) {
-    return r.tpb == 1024 || (r.chain >= 0 && r.Lp >= gs_side_level);
-  };
+  // the side stream takes the 1024-thread items (the split items of the
+  // largest slabs among them), about half the work on the Vlasov operator;
+  // they come first, and gs_apply may run the last gs_moved of them on the
+  // main stream
+  auto on_side = [](item_rec const &r) { return r.tpb == 1024; };
   auto cls = [&](item_rec const &r) {
     const int side = on_side(r) ? 0 : 1;
     return ((side * 2 + (r.tpb == 1024 ? 0 : 1)) * 2 + (r.lds > 32 * 1024 ? 0 : 1)) * 2 +
@@ -1855,6 +1891,8 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     return cls(u) != cls(v) ? cls(u) < cls(v) : u.lds > v.lds;
   });
   g.gs_launches.clear();
+  g.gs_item_lds.clear();
+  g.gs_side_items = 0;
   // partials of split bundles: per bundle ncell * chains of N values
   const size_t nbundles = bundles.size() / (sizeof(gs_bundle) / sizeof(int));
   std::vector<int64_t> part0(nbundles, -1);
@@ -1917,7 +1955,13 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     auto &l = g.gs_launches.back();
     l.lds = std::max(l.lds, irec[i].lds);
     l.count++;
+    g.gs_item_lds.push_back(irec[i].lds);
+    if (on_side(irec[i]))
+      g.gs_side_items = static_cast<int>(i) + 1;
   }
+  // until global_autotune measures it, a quarter of the side items run on
+  // the main stream (near the best split of the FP64 Vlasov operator)
+  g.gs_moved = g.gs_side_items / 4;
   // gk_sum's view of split bundles (every chunk of a split bundle is split):
   // per group and active cell the first of its chain partials
   if (nparts > 0)
@@ -1964,6 +2008,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     AKM_TRY(g.gs_lv.upload(lv.data(), int64_t(lv.size())));
     AKM_TRY(g.gs_ctab.allocate(cbase[L + 1] * esize));
     g.gs_ctab_count = cbase[L + 1];
+    g.gs_ctab_valid = false;
     g.gs_levels = L + 1;
   }
 

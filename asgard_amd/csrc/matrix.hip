@@ -262,6 +262,8 @@ akm_status akm_set_stream(akm_matrix mat, void *stream)
   if (mat == nullptr)
     return set_error(AKM_ERR_INVALID, "null matrix");
   mat->stream = static_cast<hipStream_t>(stream);
+  // tables derived from vA are rebuilt on the new stream
+  mat->vA_stamp++;
   return AKM_SUCCESS;
 }
 
@@ -470,6 +472,7 @@ akm_status akm_update_coefficients(akm_matrix mat, const void *values)
 {
   if (mat == nullptr || values == nullptr)
     return set_error(AKM_ERR_INVALID, "update coefficients: null argument");
+  mat->vA_stamp++;
   if (mat->mode == mode_dense)
   {
     const void *const *terms = static_cast<const void *const *>(values);

@@ -24,6 +24,9 @@ struct akm_matrix_s
   akm::dev_buffer<int> indx, iA;
   akm::dev_buffer<unsigned char> vA;
   int64_t num_vA = 0;
+  // advanced by every call that rewrites vA, so tables derived from it know
+  // when to rebuild
+  uint64_t vA_stamp = 0;
   // grid mode
   akm::dev_buffer<int> elem, off1d;
   int M = 0;

@@ -135,6 +135,7 @@ akm_status akm_update_operators(akm_matrix mat, const void *const *ops, int64_t 
     return AKM_SUCCESS;
   if (ops == nullptr)
     return set_error(AKM_ERR_INVALID, "operator update: null operator list");
+  mat->vA_stamp++;
   const int nops = mat->T * mat->d;
   for (int i = 0; i < nops; i++)
     if (ops[i] == nullptr)

@@ -149,3 +149,51 @@ def test_update_errors():
                                           c["iA"], c["vA"])
     with pytest.raises(asgard_amd.AkmError):
         ia.update_operators(ops)
+
+
+def test_global_slab_tables_follow_updates():
+    # the slab layout applies compact factor tables derived from the stored
+    # blocks; an operator update, a coefficient update and a stream change
+    # each make the next apply rebuild them
+    from tests.kron_cases import grid_case_ops
+
+    d, n, L, T = 4, 2, 5, 4
+    act = [{0}, {1, 2}, {0, 3}, {1, 2}]
+    ident = tuple((t, k) for t, a in enumerate(act) for k in range(d) if k not in a)
+    c, _ = grid_case_ops(d, n, L, T, seed=41, identity=ident)
+    g = asgard_amd.KronmultMatrix.global_grid(d, n, c["table"], L, T, c["vA"],
+                                              active=c["active"])
+    g.set_tuning(waves_per_row=6)
+    assert g.slab_info()[0] == 3
+    x = dev(c["x"])
+    y0 = torch.zeros_like(x)
+    g.apply(1.0, x, 0.0, y0)
+    ops2 = new_ops(c, seed=42)
+    vA2 = oracle.build_vA(d, n, T, L, ops2)
+    g.update_operators(dev_ops(ops2))
+    ref = asgard_amd.KronmultMatrix.global_grid(d, n, c["table"], L, T, vA2,
+                                                active=c["active"])
+    ref.set_tuning(waves_per_row=6)
+    y, yr = torch.zeros_like(x), torch.zeros_like(x)
+    g.apply(1.0, x, 0.0, y)
+    ref.apply(1.0, x, 0.0, yr)
+    assert torch.equal(y, yr)
+    want = global_kron.apply(d, n, c["table"], L, T, vA2, 1.0, c["x"], 0.0,
+                             np.zeros_like(c["x"]), active=c["active"])
+    assert rel(y.cpu().numpy(), want) < TOL64
+    # back to the first blocks through a coefficient update
+    g.update_stored_coefficients(c["vA"])
+    y1 = torch.zeros_like(x)
+    g.apply(1.0, x, 0.0, y1)
+    assert torch.equal(y1, y0)
+    # on another stream
+    s = torch.cuda.Stream()
+    g.set_stream(s.cuda_stream)
+    g.update_stored_coefficients(vA2)
+    y2 = torch.zeros_like(x)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        g.apply(1.0, x, 0.0, y2)
+    s.synchronize()
+    g.set_stream(0)
+    assert torch.equal(y2, yr)
