@@ -632,10 +632,15 @@ __device__ __forceinline__ void gs_block_fma(const P (&av)[NT * NT],
   }
 }
 
-//! The factor block and the x block of one list entry.
+//! Whether a cell's blocks sit value-major in LDS (FP64) or entry-major.
+template<typename P>
+constexpr bool gs_value_major = sizeof(P) == 8;
+
+//! The factor block and the x block of one list entry: the block's values
+//! are vs apart from in + ob + the entry's cell offset.
 template<typename P, int NT, int ND>
 __device__ __forceinline__ void gs_block_load(const P *tab, const P *in, unsigned e, int ob,
-                                              P (&av)[NT * NT],
+                                              int vs, P (&av)[NT * NT],
                                               P (&xv)[(ND == 1) ? NT : NT * NT])
 {
   constexpr int ds = (ND == 1) ? NT : NT * NT;
@@ -646,11 +651,12 @@ __device__ __forceinline__ void gs_block_load(const P *tab, const P *in, unsigne
     av[k] = A[k];
 #pragma unroll
   for (int k = 0; k < ds; k++)
-    xv[k] = xb[k];
+    xv[k] = xb[k * vs];
 }
 
 //! One pass along slab dimension J for one block (cell c >= 0, block base
-//! ob within the cell): the fill part of the staged factor applied to `in`.
+//! ob within the cell, values vs apart): the fill part of the staged factor
+//! applied to `in`.
 //! List entries hold the connected cell's element offset in `in` (low 16
 //! bits) and its factor block's offset in `tab` (high 16 bits).  Each
 //! value's sum runs over the connections ascending, then the factor's
@@ -661,7 +667,7 @@ __device__ __forceinline__ void gs_block_load(const P *tab, const P *in, unsigne
 template<typename P, int NT, int ND, int J>
 __device__ __forceinline__ void gs_pass_block(const P *in, const P *tab, const int *st,
                                               const int *dg, const unsigned *ent, int c, int ob,
-                                              int fill, P *v)
+                                              int vs, int fill, P *v)
 {
   constexpr int ds = (ND == 1) ? NT : NT * NT;
   const int d0 = dg[c];
@@ -670,17 +676,17 @@ __device__ __forceinline__ void gs_pass_block(const P *in, const P *tab, const i
   P av[NT * NT], xv[ds];
   if (fill == gk_upper)
   {
-    gs_block_load<P, NT, ND>(tab, in, ent[d0], ob, av, xv);
+    gs_block_load<P, NT, ND>(tab, in, ent[d0], ob, vs, av, xv);
     gs_block_fma<P, NT, ND, J, gs_from_diag>(av, xv, v);
   }
   if (lo < hi)
   {
-    gs_block_load<P, NT, ND>(tab, in, ent[lo], ob, av, xv);
+    gs_block_load<P, NT, ND>(tab, in, ent[lo], ob, vs, av, xv);
     unsigned en = ent[(lo + 1 < hi) ? lo + 1 : d0];
     for (int p = lo; p < hi; p++)
     {
       P an[NT * NT], xn[ds];
-      gs_block_load<P, NT, ND>(tab, in, en, ob, an, xn);
+      gs_block_load<P, NT, ND>(tab, in, en, ob, vs, an, xn);
       en = ent[(p + 2 < hi) ? p + 2 : d0];
       gs_block_fma<P, NT, ND, J, gs_all>(av, xv, v);
 #pragma unroll
@@ -693,7 +699,7 @@ __device__ __forceinline__ void gs_pass_block(const P *in, const P *tab, const i
   }
   if (fill == gk_lower)
   {
-    gs_block_load<P, NT, ND>(tab, in, ent[d0], ob, av, xv);
+    gs_block_load<P, NT, ND>(tab, in, ent[d0], ob, vs, av, xv);
     gs_block_fma<P, NT, ND, J, gs_before_diag>(av, xv, v);
   }
 }
@@ -736,7 +742,12 @@ __global__ void __launch_bounds__(TPB)
   const bool mine = blk < b.ncell * C;
   const int c = mine ? blk / C : 0;
   const int o = blk - (blk / C) * C;
-  const int ob = ds * o;
+  // FP64 blocks are stored value-major within the cell (value si of chunk
+  // entry o at si * C + o): a wavefront's lanes then read consecutive words
+  // instead of 32-byte strides, which spreads them over the LDS banks; FP32
+  // keeps entry-major order, where a block is one 16-byte read
+  const int vs = gs_value_major<P> ? C : 1;
+  const int ob = gs_value_major<P> ? o : ds * o;
   const int gc = mine ? a.cells[b.cell0 + c] : -1;
   const bool active = gc >= 0 && gc < a.active_cells;
   const int64_t gbase = active ? int64_t(gc) * a.N + a.offs[b.off + o] : 0;
@@ -757,7 +768,7 @@ __global__ void __launch_bounds__(TPB)
   {
 #pragma unroll
     for (int si = 0; si < ds; si++)
-      X[c * D + ob + si] = xv[si];
+      X[c * D + ob + si * vs] = xv[si];
   }
   __syncthreads(); // the pass list is read before the first pass's barrier
 
@@ -794,14 +805,14 @@ __global__ void __launch_bounds__(TPB)
       for (int si = 0; si < ds; si++)
         v[si] = 0;
       if (ND == 1 || ps.j == 0)
-        gs_pass_block<P, NT, ND, 0>(in, tb, st, dg, ent, c, ob, ps.fill, v);
+        gs_pass_block<P, NT, ND, 0>(in, tb, st, dg, ent, c, ob, vs, ps.fill, v);
       else
-        gs_pass_block<P, NT, ND, 1>(in, tb, st, dg, ent, c, ob, ps.fill, v);
+        gs_pass_block<P, NT, ND, 1>(in, tb, st, dg, ent, c, ob, vs, ps.fill, v);
 #pragma unroll
       for (int si = 0; si < ds; si++)
       {
         if (ps.mode == 1)
-          Z[c * D + ob + si] = v[si];
+          Z[c * D + ob + si * vs] = v[si];
         else
           acc[si] += v[si];
       }
