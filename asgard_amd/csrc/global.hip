@@ -1142,7 +1142,8 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
   }
   // the side launches come first and run on the side stream; gk_sum
   // follows the join
-  const bool fork = g.gs_side != nullptr;
+  // with every side item moved there is nothing to fork
+  const bool fork = g.gs_side != nullptr && g.gs_moved < g.gs_side_items;
   if (fork)
   {
     AKM_HIP(hipEventRecord(g.gs_fork, m.stream));
@@ -1440,7 +1441,8 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
   // first stage (shared input reads against fewer, longer wavefronts), each
   // cell layout with and without the XCD-contiguous workgroup order, then
   // the slab layout (when some term has one or two non-identity dimensions)
-  // with 0-80 % of its side-stream items moved to the main stream
+  // with 0-100 % of its side-stream items moved to the main stream (100:
+  // one stream, no fork and join)
   const int shapes[7][4] = {{1, 0, 0, 0}, {8, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 1, 0},
                             {0, 1, 0, 0}, {0, 1, 1, 0}, {0, 0, 0, 1}};
   for (auto const &shape : shapes)
@@ -1459,8 +1461,8 @@ int global_autotune(akm_matrix_s &m, const void *x, void *y)
     g.first_grouped = grouped;
     g.xcd_order = xcd;
     g.slab = slab;
-    const int percents[8] = {0, 10, 20, 30, 40, 50, 65, 80};
-    const int nmoves = (slab && g.gs_side != nullptr) ? 8 : 1;
+    const int percents[9] = {0, 10, 20, 30, 40, 50, 65, 80, 100};
+    const int nmoves = (slab && g.gs_side != nullptr) ? 9 : 1;
     for (int mi = 0; mi < nmoves && rc == AKM_SUCCESS; mi++)
     {
       g.gs_moved = g.gs_side_items * percents[mi] / 100;
