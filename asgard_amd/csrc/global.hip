@@ -475,6 +475,7 @@ constexpr int gs_blocks_small = 256;
 constexpr int gs_blocks_large = 1024;
 constexpr int gs_lds_max = 160 * 1024;
 constexpr int gs_pad = 2; // values of padding after each cell's blocks in LDS
+static_assert(gs_pad % 2 == 0, "pair-major FP64 blocks need cells an even count of values apart");
 constexpr int gs_split_level = 5; // slabs with level bound >= this run one chain per workgroup
 constexpr int gs_lds_resident = 64 * 1024; // stage everything at once up to this size
 constexpr int gs_unroll = 8;               // independent loads per thread while staging
@@ -632,12 +633,33 @@ __device__ __forceinline__ void gs_block_fma(const P (&av)[NT * NT],
   }
 }
 
-//! Whether a cell's blocks sit value-major in LDS (FP64) or entry-major.
+//! How a cell's blocks of DS values sit in LDS.  FP64 blocks with an even
+//! value count are pair-major: values 2q and 2q + 1 of chunk entry o at
+//! 2 (q C + o), one 16-byte read.  Other FP64 blocks are value-major (value
+//! s of entry o at s C + o).  FP32 blocks are entry-major (DS o + s).  In
+//! the FP64 orders a wavefront's lanes, which hold consecutive entries o,
+//! read consecutive words instead of 32-byte strides, which spreads them
+//! over the LDS banks; an FP32 block of four values is one 16-byte read.
 template<typename P>
 constexpr bool gs_value_major = sizeof(P) == 8;
+template<typename P, int DS>
+constexpr bool gs_pair_major = sizeof(P) == 8 && DS % 2 == 0;
 
-//! The factor block and the x block of one list entry: the block's values
-//! are vs apart from in + ob + the entry's cell offset.
+//! The LDS offset of value si of a block whose entry starts at ob, with
+//! value (pair, for pair-major blocks) stride vs.
+template<typename P, int DS>
+__device__ __forceinline__ int gs_at(int si, int ob, int vs)
+{
+  if constexpr (gs_pair_major<P, DS>)
+    return ob + (si >> 1) * vs + (si & 1);
+  else
+    return ob + si * vs;
+}
+
+//! The factor block and the x block of one list entry: the block starts at
+//! in + ob + the entry's cell offset, its values (pairs, when pair-major)
+//! vs apart.  Factor blocks of an even value count are read in 16-byte
+//! pairs (FP64 tables start 16-byte aligned, blocks n*n values apart).
 template<typename P, int NT, int ND>
 __device__ __forceinline__ void gs_block_load(const P *tab, const P *in, unsigned e, int ob,
                                               int vs, P (&av)[NT * NT],
@@ -646,17 +668,43 @@ __device__ __forceinline__ void gs_block_load(const P *tab, const P *in, unsigne
   constexpr int ds = (ND == 1) ? NT : NT * NT;
   const P *A = tab + (e >> 16);
   const P *xb = in + (e & 0xffffu) + ob;
+  if constexpr (gs_pair_major<P, NT * NT>)
+  {
 #pragma unroll
-  for (int k = 0; k < NT * NT; k++)
-    av[k] = A[k];
+    for (int q = 0; q < NT * NT / 2; q++)
+    {
+      const double2 t = reinterpret_cast<const double2 *>(A)[q];
+      av[2 * q] = t.x;
+      av[2 * q + 1] = t.y;
+    }
+  }
+  else
+  {
 #pragma unroll
-  for (int k = 0; k < ds; k++)
-    xv[k] = xb[k * vs];
+    for (int k = 0; k < NT * NT; k++)
+      av[k] = A[k];
+  }
+  if constexpr (gs_pair_major<P, ds>)
+  {
+#pragma unroll
+    for (int q = 0; q < ds / 2; q++)
+    {
+      const double2 t = *reinterpret_cast<const double2 *>(xb + q * vs);
+      xv[2 * q] = t.x;
+      xv[2 * q + 1] = t.y;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < ds; k++)
+      xv[k] = xb[k * vs];
+  }
 }
 
 //! One pass along slab dimension J for one block (cell c >= 0, block base
-//! ob within the cell, values vs apart): the fill part of the staged factor
-//! applied to `in`.
+//! ob within the cell, values or pairs vs apart): the fill part of the
+//! staged factor applied to `in`.
 //! List entries hold the connected cell's element offset in `in` (low 16
 //! bits) and its factor block's offset in `tab` (high 16 bits).  Each
 //! value's sum runs over the connections ascending, then the factor's
@@ -742,12 +790,12 @@ __global__ void __launch_bounds__(TPB)
   const bool mine = blk < b.ncell * C;
   const int c = mine ? blk / C : 0;
   const int o = blk - (blk / C) * C;
-  // FP64 blocks are stored value-major within the cell (value si of chunk
-  // entry o at si * C + o): a wavefront's lanes then read consecutive words
-  // instead of 32-byte strides, which spreads them over the LDS banks; FP32
-  // keeps entry-major order, where a block is one 16-byte read
-  const int vs = gs_value_major<P> ? C : 1;
-  const int ob = gs_value_major<P> ? o : ds * o;
+  // where this entry's block starts within a cell, and its value (pair)
+  // stride (pair-major, value-major or entry-major, as gs_at reads them);
+  // cells are D values apart, D even for pair-major blocks (gs_pad is even),
+  // so pairs stay 16-byte aligned
+  const int vs = gs_pair_major<P, ds> ? 2 * C : (gs_value_major<P> ? C : 1);
+  const int ob = gs_pair_major<P, ds> ? 2 * o : (gs_value_major<P> ? o : ds * o);
   const int gc = mine ? a.cells[b.cell0 + c] : -1;
   const bool active = gc >= 0 && gc < a.active_cells;
   const int64_t gbase = active ? int64_t(gc) * a.N + a.offs[b.off + o] : 0;
@@ -768,7 +816,7 @@ __global__ void __launch_bounds__(TPB)
   {
 #pragma unroll
     for (int si = 0; si < ds; si++)
-      X[c * D + ob + si * vs] = xv[si];
+      X[c * D + gs_at<P, ds>(si, ob, vs)] = xv[si];
   }
   __syncthreads(); // the pass list is read before the first pass's barrier
 
@@ -812,7 +860,7 @@ __global__ void __launch_bounds__(TPB)
       for (int si = 0; si < ds; si++)
       {
         if (ps.mode == 1)
-          Z[c * D + ob + si * vs] = v[si];
+          Z[c * D + gs_at<P, ds>(si, ob, vs)] = v[si];
         else
           acc[si] += v[si];
       }
