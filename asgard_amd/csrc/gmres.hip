@@ -680,6 +680,22 @@ struct gm_cycle
     return AKM_SUCCESS;
   }
 
+  //! start() without the host's wait: the state goes to the last host slot,
+  //! which run_steps reads (with beta_pending) after enqueueing its first
+  //! steps, so the GPU runs on through the round trip.  Needs events.
+  int start_async(P tol)
+  {
+    static_assert(gm_ahead + 1 < gm_slots, "the first steps in flight reach the start's slot");
+    gm_sumsq<P><<<nblk, gm_threads, 0, s>>>(n, rpb, V, sqp);
+    gm_givens<P><<<1, 64, 0, s>>>(nblk, sqp, h, -1, H, cs, sn, g, tol, rotate_always, st);
+    gm_scale<P><<<sblocks, gm_threads, 0, s>>>(n, V, st);
+    AKM_TRY(check_launch("gmres restart"));
+    AKM_HIP(hipMemcpyAsync(hst + (gm_slots - 1), st, sizeof(gm_state<P>),
+                           hipMemcpyDeviceToHost, s));
+    AKM_HIP(hipEventRecord(ready[gm_slots - 1], s));
+    return AKM_SUCCESS;
+  }
+
   //! Arnoldi step j: z_j = pre(v_j) (flexible) or v_j, w = op(z_j), CGS
   //! against V_0..V_j, Givens; est = new residual estimate.
   int step(int j, std::function<int(const P *, P *)> const &op,
@@ -727,12 +743,21 @@ struct gm_cycle
   //! count only), and the stream runs them before any later work.
   int run_steps(int limit, std::function<int(const P *, P *)> const &op,
                 std::function<int(const P *, P *)> const *pre, P tol, P stop, P &est,
-                int &count, bool speculate)
+                int &count, bool speculate, bool beta_pending = false)
   {
     static_assert(gm_ahead + 1 <= gm_slots, "steps in flight exceed the state slots");
     count = 0;
     int launched = 0;
     const int depth = speculate ? gm_ahead + 1 : 1;
+    if (beta_pending)
+    {
+      // the first steps go out before the start's residual norm is known;
+      // when that norm already meets `stop` they are never used (count 0)
+      while (launched < depth && launched < limit)
+        AKM_TRY(launch_step(launched++, op, pre, tol));
+      AKM_HIP(hipEventSynchronize(ready[gm_slots - 1]));
+      est = hst[gm_slots - 1].inner_res;
+    }
     while (est > stop && count < limit)
     {
       while (launched < count + depth && launched < limit)
@@ -783,9 +808,15 @@ static int gmres_outer(gm_cycle<P> &cyc, R &&residual,
   {
     AKM_TRY(residual(outer, speculate));
     ++total;
-    AKM_TRY(cyc.start(tol, inner_res));
+    // with events the host reads the start's norm only once the first
+    // steps are enqueued
+    const bool pending = cyc.ready != nullptr;
+    if (pending)
+      AKM_TRY(cyc.start_async(tol));
+    else
+      AKM_TRY(cyc.start(tol, inner_res));
     int inner = 0;
-    AKM_TRY(cyc.run_steps(restart, op, nullptr, tol, tol, inner_res, inner, speculate));
+    AKM_TRY(cyc.run_steps(restart, op, nullptr, tol, tol, inner_res, inner, speculate, pending));
     total += inner;
     AKM_TRY(cyc.finish(inner, x, false));
     ++outer;
@@ -827,11 +858,18 @@ static int gmres_typed(akm_matrix_s &m, gm_memory const &mem, P dt, P *x, const 
     AKM_HIP(hipMemcpyAsync(V, b, sizeof(P) * n, hipMemcpyDeviceToDevice, s));
     if (outer == 0 && mem.ready != nullptr)
     {
-      double ms = 0;
-      AKM_TRY(timed_on_stream(s, [&]() { return apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x); },
-                              ms));
-      speculate = ms < speculate_below_ms;
-      return AKM_SUCCESS;
+      // the first solve on the handle times its first apply (a host wait);
+      // later solves reuse that time
+      if (m.gmres_apply_ms < 0)
+      {
+        double ms = 0;
+        AKM_TRY(timed_on_stream(
+            s, [&]() { return apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x); }, ms));
+        m.gmres_apply_ms = ms;
+        speculate = ms < speculate_below_ms;
+        return AKM_SUCCESS;
+      }
+      speculate = m.gmres_apply_ms < speculate_below_ms;
     }
     return apply_op(-dt * P{-1}, x, P{1}, V, P{-1}, x);
   };

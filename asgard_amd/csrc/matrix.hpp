@@ -27,6 +27,9 @@ struct akm_matrix_s
   // advanced by every call that rewrites vA, so tables derived from it know
   // when to rebuild
   uint64_t vA_stamp = 0;
+  // the first implicit solve's timed operator apply (ms; -1 before it),
+  // which decides whether GMRES steps are enqueued ahead
+  double gmres_apply_ms = -1;
   // grid mode
   akm::dev_buffer<int> elem, off1d;
   int M = 0;
