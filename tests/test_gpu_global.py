@@ -344,3 +344,17 @@ def test_slab_layout_vlasov_d6_full_size():
     if layout == 6:
         assert torch.equal(y4, y2)
     assert rel(host(y4), host(y1)) < 1e-12
+    # FP32: the autotuner moves most split workgroups to the main stream
+    g32 = asgard_amd.KronmultMatrix.global_grid(d, n, table, L, T, vA.astype(np.float32),
+                                                dtype=np.float32, active=act)
+    x32 = x.float()
+    g32.set_tuning(waves_per_row=6)
+    z1 = torch.zeros_like(x32)
+    g32.apply(1.0, x32, 0.0, z1)
+    zt = torch.zeros_like(x32)
+    layout32 = g32.autotune(x32, zt)[0]
+    z2 = torch.zeros_like(x32)
+    g32.apply(1.0, x32, 0.0, z2)
+    if layout32 == 6:
+        assert torch.equal(z1, z2)
+    assert rel(host(z2).astype(np.float64), host(y1)) < 1e-5
