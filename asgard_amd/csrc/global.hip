@@ -1934,8 +1934,9 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
       term_group[t] = -1; // its group did not fit: cell-layout stages
   g.ngroups = ngroups;
 
-  // launches: 1024-thread items, then 256-thread items above and below 32 KB
-  // of LDS (the small ones keep more workgroups per CU)
+  // launches: 1024-thread items, then 256-thread items, per slab dimension
+  // count (one launch for all 256-thread items, largest LDS first, measured
+  // faster than separate launches above and below 32 KB of LDS)
   // the side stream takes the 1024-thread items (the split items of the
   // largest slabs among them), about half the work on the Vlasov operator;
   // they come first, and gs_apply may run the last gs_moved of them on the
@@ -1943,7 +1944,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
   auto on_side = [](item_rec const &r) { return r.tpb == 1024; };
   auto cls = [&](item_rec const &r) {
     const int side = on_side(r) ? 0 : 1;
-    return ((side * 2 + (r.tpb == 1024 ? 0 : 1)) * 2 + (r.lds > 32 * 1024 ? 0 : 1)) * 2 +
+    return (side * 2 + (r.tpb == 1024 ? 0 : 1)) * 2 +
            (r.nd - 1);
   };
   // within a launch the largest items first (the longest-running workgroups
