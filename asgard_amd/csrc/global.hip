@@ -472,6 +472,14 @@ __global__ void __launch_bounds__(gk_threads)
 // a bundle holds at most 256 blocks (one per thread), a large slab alone up
 // to 1024
 constexpr int gs_blocks_small = 256;
+//! Blocks of a small bundle tried first: FP64 bundles of 512 blocks (512
+//! threads) measured faster than 256 on the d=6 Vlasov operator, FP32 ones
+//! slower.  A group whose 512-block bundles do not fit LDS is planned again
+//! with 256.
+__host__ __device__ constexpr int gs_blocks_small_first(int esize)
+{
+  return esize == 8 ? 512 : gs_blocks_small;
+}
 constexpr int gs_blocks_large = 1024;
 constexpr int gs_lds_max = 160 * 1024;
 constexpr int gs_pad = 2; // values of padding after each cell's blocks in LDS
@@ -1211,6 +1219,8 @@ static int gs_apply(akm_matrix_s &m, const P *vA, const P *x, P *work)
                                           g.gs_item_lds.begin() + first + count);
     if (tpb == 1024)
       return gs_launch_class<P, 1024>(s, m.n, nd, ga, ctab, items, count, lds, x, work, part);
+    if (tpb == 512)
+      return gs_launch_class<P, 512>(s, m.n, nd, ga, ctab, items, count, lds, x, work, part);
     return gs_launch_class<P, 256>(s, m.n, nd, ga, ctab, items, count, lds, x, work, part);
   };
   for (auto const &l : g.gs_launches)
@@ -1651,11 +1661,18 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
   std::vector<int> gslot(gdims.size(), -1);
   int ngroups = 0;
   for (size_t gi = 0; gi < gdims.size(); gi++)
+    for (int attempt = 0; attempt < 2; attempt++)
   {
+    // small bundles of gs_blocks_small_first blocks, else (second attempt,
+    // when those do not fit) of gs_blocks_small; a failed attempt commits
+    // nothing
+    const int blocks_small = (attempt == 0) ? gs_blocks_small_first(esize) : gs_blocks_small;
+    if (attempt == 1 && blocks_small == gs_blocks_small_first(esize))
+      break;
     const std::vector<int> &S = gdims[gi];
     const int nd = static_cast<int>(S.size());
     const int ds = (nd == 1) ? n : nn;
-    const int64_t gs_cap_small = int64_t(gs_blocks_small) * ds; // values
+    const int64_t gs_cap_small = int64_t(blocks_small) * ds; // values
     const int64_t gs_cap_large = int64_t(gs_blocks_large) * ds;
     if (n < 2 || n > 4)
       continue; // the slab kernels hold n^nd-value blocks in registers
@@ -1682,6 +1699,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     std::vector<gs_plan_bundle> plan;
     bool ok = true;
     gs_plan_bundle open;
+    open.tpb = blocks_small;
     int64_t open_vals = 0;
     auto slab_level = [&](std::vector<int> const &sc) {
       int lp = 0;
@@ -1702,6 +1720,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
         {
           plan.push_back(open);
           open = gs_plan_bundle();
+          open.tpb = blocks_small;
           open_vals = 0;
         }
         open.cells.insert(open.cells.end(), sc.begin(), sc.end());
@@ -1721,7 +1740,7 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
       gs_plan_bundle one;
       one.cells = sc;
       one.Lp = lp;
-      one.tpb = (cap == gs_cap_small) ? 256 : 1024;
+      one.tpb = (cap == gs_cap_small) ? blocks_small : 1024;
       one.C = call;
       while (msz * ds * one.C > cap)
         one.C /= n;
@@ -1928,15 +1947,17 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
     irec.insert(irec.end(), p_items.begin(), p_items.end());
     gslot[gi] = group_index;
     ngroups++;
+    break;
   }
   for (int t = 0; t < T; t++)
     if (term_group[t] >= 0 && gslot[term_group[t]] < 0)
       term_group[t] = -1; // its group did not fit: cell-layout stages
   g.ngroups = ngroups;
 
-  // launches: 1024-thread items, then 256-thread items, per slab dimension
-  // count (one launch for all 256-thread items, largest LDS first, measured
-  // faster than separate launches above and below 32 KB of LDS)
+  // launches: 1024-thread items, then the small bundles' (512 or 256
+  // threads), per slab dimension count (one launch for all items of a size,
+  // largest LDS first, measured faster than separate launches above and
+  // below 32 KB of LDS)
   // the side stream takes the 1024-thread items (the split items of the
   // largest slabs among them), about half the work on the Vlasov operator;
   // they come first, and gs_apply may run the last gs_moved of them on the
@@ -1944,8 +1965,8 @@ static int build_slabs(akm_matrix_s &m, akm_global_s &g, std::vector<gk_lists> c
   auto on_side = [](item_rec const &r) { return r.tpb == 1024; };
   auto cls = [&](item_rec const &r) {
     const int side = on_side(r) ? 0 : 1;
-    return (side * 2 + (r.tpb == 1024 ? 0 : 1)) * 2 +
-           (r.nd - 1);
+    const int size = (r.tpb == 1024) ? 0 : (r.tpb == 512 ? 1 : 2);
+    return (side * 3 + size) * 2 + (r.nd - 1);
   };
   // within a launch the largest items first (the longest-running workgroups
   // start early)
