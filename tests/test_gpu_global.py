@@ -238,6 +238,8 @@ SLAB_CASES = [
     # in such a slab of every group, each group's chain partials kept apart
     dict(d=6, n=2, L=5, act=[{0, 3}, {1, 4}, {2, 5}, {0, 3}, {1, 4}, {2, 5}]),
     dict(d=5, n=2, L=4, act=[{4}, {0, 4}, {1, 3}, {0, 1, 2, 3, 4}], refine=0.3),
+    # n = 4 two-dimension slabs: FP64 bundles of 512 blocks exceed LDS and the
+    # group is planned again with 256
     dict(d=3, n=4, L=5, act=[{0, 1}, {2}]),
     dict(d=2, n=4, L=4, act=[{0, 1}, {1}]),
     dict(d=1, n=3, L=5, act=[{0}, {0}]),
