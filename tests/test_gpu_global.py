@@ -247,6 +247,10 @@ SLAB_CASES = [
     dict(d=2, n=2, L=7, act=[{0, 1}, {0}]),
     # level-9 factor tables do not fit LDS: every term runs as layout 2
     dict(d=2, n=2, L=9, act=[{0, 1}, {1}]),
+    # refined grids reaching levels 6 and 7: split slabs (one chain per
+    # workgroup) on grids that are not lower sets
+    dict(d=4, n=2, L=6, act=[{0, 2}, {1, 3}, {0, 2}], refine=0.3),
+    dict(d=3, n=2, L=7, act=[{0, 1}, {2}, {1, 2}], refine=0.3),
 ]
 
 
